@@ -1,0 +1,170 @@
+#!/usr/bin/env python
+"""Throughput of the Grains3D DEM time step on one MI355X.
+
+Workload: 1,000,000 polydisperse spheres, radii U[0.8, 1.2] mm, triply
+periodic cube at solid fraction 0.3, Gaussian velocities (sigma 0.1 m/s), no
+gravity, Hooke en = 0.8, mu = 0.3, kn = 1e4, dt = 1e-5 (homogeneous cooling
+granular gas), SecondOrderLeapFrog.  The headline value is the mixed path
+(FP32 force/torque arithmetic, FP64 positions, quaternions and velocities);
+the plain FP64 path is measured on the same state and reported beside it.
+
+A step is one full time step (kick, drift, re-binning, contact detection,
+forces, kick) over every particle; value = N * K / (elapsed time of K steps),
+state resident in HBM, timed with HIP events on the library's stream.
+
+Prints one JSON line.  Only N = 1 GPU is supported.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle-step: state read once and written once per
+# step.  FP64: x, v, w, q read+write (13 f64 x 8 B x 2 = 208 B), radius read
+# (8 B), F and T carried between the half kicks (6 f64 read+write = 96 B),
+# cell key write+read (8 B) -> 320 B; mixed: F, T in FP32 -> 272 B; plus 16 B
+# per contact pair (one int32 pair written and read).
+B_ALG = {"fp64": 320.0, "mixed": 272.0}
+B_PAIR = 16.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_gpu(wl, precision):
+    from pacific_amd import GrainsGPU, workloads
+    g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], 0.0, wl["dt"],
+                  0, precision)
+    workloads.build(g, wl)
+    g.init()
+    return g
+
+
+def contacts_per_particle(g):
+    g.record_pairs(True)
+    g.compute_forces()
+    n_rec = len(g.pairs())  # directed records = 2 per particle-particle contact
+    g.record_pairs(False)
+    return n_rec / 2.0 / g.n
+
+
+def cpu_baseline(wl, steps):
+    """The CPU oracle (serial FP64 restatement of the reference step, one
+    core) on the same workload for a bounded number of steps."""
+    from oracle.oracle import Oracle
+    from pacific_amd import workloads
+    o = Oracle(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], 0.0, wl["dt"], 0)
+    workloads.build(o, wl)
+    o.init()
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    return wl["radius"].size * steps / (t1 - t0), t1 - t0
+
+
+def read_traffic(name):
+    """HBM bytes per launch of the force sweep from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=400)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the DEM step on one GPU only (no sharding, no collectives); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from pacific_amd import FP64, MIXED, workloads
+    wl = workloads.periodic_gas(args.n, seed=1)
+    prec = MIXED if args.precision == "mixed" else FP64
+    log(f"bench: building {wl['name']} ({args.precision})")
+    g = build_gpu(wl, prec)
+    g.step(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    ms = g.step_timed(args.steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    value = args.n * args.steps / (ms / 1e3)
+    log(f"bench: {args.steps} steps in {ms:.1f} ms (wall {wall:.3f} s) -> {value:.4g} particle-steps/s")
+
+    cbar = contacts_per_particle(g)
+    prof = g.profile_steps(args.profile_steps)
+    force_ms = prof["force_ms"] / max(prof["n_force"], 1)
+    bin_ms = prof["bin_ms"] / max(prof["n_bin"], 1)
+    b_alg = (B_ALG[args.precision] + B_PAIR * cbar) * args.n
+    achieved = b_alg / (force_ms / 1e3) / 1e9
+    log(f"bench: force sweep {force_ms * 1e3:.1f} us/launch, binning {bin_ms * 1e3:.1f} us/step, "
+        f"contacts/particle {cbar:.4f}")
+
+    extra = {}
+    if not args.no_fp64 and prec != FP64:
+        g64 = build_gpu(wl, FP64)
+        g64.step(args.warmup)
+        ms64 = g64.step_timed(max(args.steps // 2, 1))
+        extra["fp64_value"] = args.n * max(args.steps // 2, 1) / (ms64 / 1e3)
+        extra["fp64_ms_per_step"] = ms64 / max(args.steps // 2, 1)
+        del g64
+        log(f"bench: FP64 path {extra['fp64_value']:.4g} particle-steps/s")
+
+    traffic = read_traffic(f"traffic_forces_{args.precision}_{args.n}.json")
+    out = {
+        "metric": "particle-steps/s on 1 GPU, plain FP64 vs mixed/batched; adjoint/forward cost ratio",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-force/f64-state" if prec == MIXED else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"periodic cooling granular gas, {args.n} polydisperse spheres r~U[0.8,1.2] mm, "
+                               "phi=0.3, sigma_v=0.1 m/s, Hooke kn=1e4 en=0.8 mu=0.3, dt=1e-5, LeapFrog",
+                   "particles": args.n, "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"k_forces<fused,{args.precision}>", "kernel_ms": force_ms,
+                     "bytes_alg_per_launch": b_alg, "contacts_per_particle": cbar, "binning_ms": bin_ms},
+    }
+    out.update(extra)
+    if not args.no_cpu:
+        cpu_value, cpu_s = cpu_baseline(wl, args.cpu_steps)
+        out["cpu_baseline"] = {"value": cpu_value, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"the same {args.n}-particle gas, {args.cpu_steps} steps "
+                                         f"({cpu_s:.1f} s), FP64 serial oracle restating the reference step"}
+        log(f"bench: CPU oracle {cpu_value:.4g} particle-steps/s on 1 core")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * grains_gpu.h -- C ABI of the MI355X DEM time step (libgrains_gpu.so).
+ *
+ * Plain C types only: pointers, sizes, doubles and ints.  All arrays are
+ * host memory, row-major, ordered by particle id (ids are 1..N in insertion
+ * order, obstacles are -1..-M), so that a caller written against Grains3D's
+ * Particle / Obstacle / ContactForceModel objects can fill them directly.
+ *
+ * Every call returns GRAINS_OK (0) or an error code; grains_gpu_last_error()
+ * gives the message.  Errors raised inside the time step (ContactError,
+ * MotionError, SimulationError of the reference, GRAINS_V1/Grains/Base/
+ * include/Error.hh) are flagged on the device and returned by the next call
+ * that synchronises (grains_gpu_step, grains_gpu_synchronize, the getters).
+ *
+ * A system is not thread safe; distinct systems are independent and may run
+ * on distinct threads.
+ */
+#ifndef GRAINS_GPU_H
+#define GRAINS_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct grains_gpu_system grains_gpu_system;
+
+/* status codes; 1-3 mirror the reference's ContactError / MotionError /
+ * SimulationError (Grains.cpp:423-450) */
+enum {
+  GRAINS_OK = 0,
+  GRAINS_CONTACT_ERROR = 1,
+  GRAINS_MOTION_ERROR = 2,
+  GRAINS_SIMULATION_ERROR = 3,
+  GRAINS_CONFIG_ERROR = 4,
+  GRAINS_DEVICE_ERROR = 5,
+  GRAINS_ARGUMENT_ERROR = 6
+};
+
+/* arithmetic of the force model */
+enum { GRAINS_FP64 = 0, GRAINS_MIXED = 1 };
+
+/* time integrators, <TimeIntegration Type=...> (TimeIntegratorBuilderFactory.cpp:13-40) */
+enum {
+  GRAINS_LEAPFROG = 0,               /* "SecondOrderLeapFrog" (default) */
+  GRAINS_FIRST_ORDER_EXPLICIT = 1,   /* "FirstOrderExplicit" */
+  GRAINS_SECOND_ORDER_EXPLICIT = 2,  /* "SecondOrderExplicit" */
+  GRAINS_ADAMS_BASHFORTH2 = 3        /* "SecondOrderAdamsBashforth" */
+};
+
+typedef struct {
+  double origin[3];        /* <Origin OX OY OZ> */
+  double max[3];           /* <LinkedCell MX MY MZ> (maximum coordinates) */
+  int periodic[3];         /* <Periodicity PX PY PZ> */
+  double cell_size_factor; /* <LinkedCell CellSizeFactor> (>= 1) */
+  double gravity[3];       /* <Forces><Gravity GX GY GZ> */
+  double fluid_density;    /* 0 for a dry simulation */
+  double dt;               /* <TimeStep dt> */
+  double t_start;          /* <TimeInterval Start> */
+  int scheme;              /* GRAINS_LEAPFROG ... */
+  int precision;           /* GRAINS_FP64 or GRAINS_MIXED */
+  int device;              /* HIP device ordinal */
+} grains_gpu_config;
+
+/* Replaces Grains::Construction / Forces / AdditionalFeatures setup of an
+ * empty system (GRAINS_V1/Grains/Base/src/Grains.cpp:535-1100). */
+int grains_gpu_create(const grains_gpu_config* cfg, grains_gpu_system** out);
+
+/* Builds a system from a Grains3D XML input file (the <Construction>,
+ * <Forces> and <Simulation> sections, Grains.cpp:95-179).  Particles listed
+ * through <ParticlePosition><File> or <StructuredArray> are inserted at the
+ * initial time; the system is initialised (grains_gpu_initialize) on return.
+ * Replaces GrainsBuilderFactory::create + Grains::do_before_time_stepping. */
+int grains_gpu_create_from_xml(const char* xml_path, int precision, grains_gpu_system** out);
+
+void grains_gpu_destroy(grains_gpu_system* sys);
+
+/* Hooke contact model of a material pair; matB < 0 sets the Default model.
+ * Parameters of <Hooke><kn/><en/><muc/><kr/>[<etat/>]</Hooke>
+ * (HookeContactForceModel.cpp:170-199); etat = -1 selects the default
+ * tangential damping.  Replaces ContactBuilderFactory::define
+ * (ContactBuilderFactory.cpp:82-117). */
+int grains_gpu_set_contact_model(grains_gpu_system* sys, int matA, int matB, double kn, double en, double etat,
+                                 double muc, double kr);
+
+/* Adds n spheres in insertion order (ids continue from the current count).
+ * radius, density, crust: n values each; material: n ints; x, v, w: 3n
+ * values (w = space-fixed angular velocity); v and w may be NULL (zero).
+ * Replaces Particle(DOMNode*) + Grains::insertParticle
+ * (Particle.cpp:150-166, Grains.cpp:2070-2140). */
+int grains_gpu_add_spheres(grains_gpu_system* sys, size_t n, const double* radius, const double* density,
+                           const double* crust, const int* material, const double* x, const double* v,
+                           const double* w);
+
+/* Adds a static box obstacle: edge lengths L[3], centre[3], basis[9] (row
+ * major rotation, NULL = identity).  Replaces SimpleObstacle(DOMNode*) with
+ * a <Box LX LY LZ> convex (SimpleObstacle.cpp:40-90, Box.cpp:52-60). */
+int grains_gpu_add_box(grains_gpu_system* sys, const double* L, const double* centre, const double* basis,
+                       double crust, int material);
+
+/* Builds the linked-cell grid, uploads the state, bins it and (LeapFrog)
+ * computes the initial forces (Grains.cpp:95-179, 2522-2527). */
+int grains_gpu_initialize(grains_gpu_system* sys);
+
+/* Runs nsteps time steps of Grains::Simulation's loop body
+ * (Grains.cpp:286-452) without output. */
+int grains_gpu_step(grains_gpu_system* sys, int nsteps);
+
+/* Same, timed with HIP events on the system's stream; *ms = elapsed time. */
+int grains_gpu_step_timed(grains_gpu_system* sys, int nsteps, double* ms);
+
+/* Same, with HIP events around every force sweep and every binning sequence
+ * on the system's stream: out[0] total ms, out[1] summed force-sweep ms,
+ * out[2] number of force sweeps, out[3] summed binning ms, out[4] number of
+ * binning sequences.  No counterpart in the reference (instrumentation of
+ * the Grains::Simulation timers, solvercomputingtime.cpp:48-215). */
+int grains_gpu_profile_steps(grains_gpu_system* sys, int nsteps, double* out);
+
+int grains_gpu_synchronize(grains_gpu_system* sys);
+
+/* Forces and torques at the current state (Grains::computeParticlesForceAndTorque,
+ * Grains.cpp:459-482); results via grains_gpu_get_state(F, T). */
+int grains_gpu_compute_forces(grains_gpu_system* sys);
+
+/* State in id order: x, v, w, wb, F, T: 3 doubles per particle; q: 4 (x, y,
+ * z, w).  NULL pointers are skipped. */
+int grains_gpu_get_state(grains_gpu_system* sys, double* x, double* v, double* w, double* wb, double* q,
+                         double* F, double* T);
+
+/* Overwrites positions, velocities, angular velocities and quaternions (id
+ * order; NULL keeps the current value), re-bins, and recomputes the forces
+ * when recompute_forces != 0 (the LeapFrog state after a reload). */
+int grains_gpu_set_state(grains_gpu_system* sys, const double* x, const double* v, const double* w,
+                         const double* q, int recompute_forces);
+
+/* mass, sphere moment of inertia and weight (3 per particle) as built from
+ * radius and density (Particle.cpp:157-166, 413-418). */
+int grains_gpu_get_masses(grains_gpu_system* sys, double* mass, double* inertia, double* weight);
+
+/* Contact records of the last force evaluation: pairs (receiver id, partner
+ * id), one per particle that received a contact force from the partner
+ * (obstacle partners have negative ids).  Recording is off by default. */
+int grains_gpu_set_record_pairs(grains_gpu_system* sys, int on);
+int grains_gpu_get_pairs(grains_gpu_system* sys, int* buf, size_t cap, size_t* n);
+
+size_t grains_gpu_num_particles(const grains_gpu_system* sys);
+double grains_gpu_time(const grains_gpu_system* sys);
+void grains_gpu_set_time(grains_gpu_system* sys, double t);
+/* grid cells per axis (GPU interior cells) */
+int grains_gpu_grid(const grains_gpu_system* sys, int* dims, double* edge);
+/* ids of the bodies named by the last device-side error */
+int grains_gpu_error_ids(grains_gpu_system* sys, int* id0, int* id1);
+/* number of ContactError recoveries (RigidBodyWithCrust.cpp:534-590) */
+int grains_gpu_recoveries(grains_gpu_system* sys, long* n);
+const char* grains_gpu_last_error(const grains_gpu_system* sys);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAINS_GPU_H */

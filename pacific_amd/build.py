@@ -1,0 +1,65 @@
+"""Build recipe of the in-tree HIP library ``pacific_amd/libgrains_gpu.so``.
+
+Compiles every source under ``pacific_amd/csrc`` for gfx950 with hipcc.  The
+FP64 path must evaluate expressions in source order (``-ffp-contract=off``);
+packed FP32 instructions are disabled for all HIP code.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgrains_gpu.so")
+CLI = os.path.join(HERE, "grains_gpu")
+
+SOURCES = ["dem_kernels.hip", "dem_system.cpp", "grains_xml.cpp"]
+ARCH = os.environ.get("PACIFIC_AMD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    "-O3",
+    "-std=c++17",
+    f"--offload-arch={ARCH}",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build(verbose=False):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    out = _run([hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+    cli_src = os.path.join(CSRC, "grains_cli.cpp")
+    if os.path.exists(cli_src):
+        tmpc = CLI + ".tmp"
+        out += _run([hipcc()] + HIPCC_FLAGS + ["-o", tmpc, cli_src, "-L" + HERE, "-lgrains_gpu",
+                                               "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmpc, CLI)
+    if verbose and out.strip():
+        print(out)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    print(LIB)
+    sys.exit(0)

@@ -1,0 +1,27 @@
+// Host entry points of the DEM kernels (dem_kernels.hip)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dem_types.hpp"
+
+namespace dem {
+
+enum ForceMode { kModeForcesOnly = 0, kModeForcesKick = 1, kModeFused = 2 };
+
+// first half kick + drift (or a full Move of the explicit schemes) in place,
+// then cell key and arrival slot
+hipError_t launch_move(const Particles& P, const Sorted& S, const Grid& G, const StepParams& sp, DevStatus* st,
+                       double dtv, double dtx, hipStream_t stream);
+// cell key and arrival slot of the current positions (no motion)
+hipError_t launch_key(const Particles& P, const Sorted& S, const Grid& G, const StepParams& sp, DevStatus* st,
+                      hipStream_t stream);
+// counts -> start, scatter, per-cell ordering; cx = position buffer to gather
+hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n, int cx, hipStream_t stream);
+int scan_blocks(int ncells);
+// zero n ints with a kernel (safe inside captured graphs, unlike hipMemsetAsync)
+hipError_t launch_clear(int* p, int n, hipStream_t stream);
+hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
+                         const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,
+                         int* pairs, hipStream_t stream);
+
+}  // namespace dem

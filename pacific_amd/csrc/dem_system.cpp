@@ -1,0 +1,720 @@
+// Host side of the MI355X DEM time step: system construction, device memory,
+// the per-step launch sequence and the C ABI declared in include/grains_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/grains_gpu.h"
+#include "dem_launch.hpp"
+#include "dem_math.hpp"
+#include "dem_system.hpp"
+
+using namespace dem;
+
+namespace {
+
+int fail(grains_gpu_system* s, int code, const std::string& msg) {
+  if (s) s->last_error = msg;
+  return code;
+}
+
+#define HIPCHECK(sys, expr)                                                                         \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess)                                                                           \
+      return fail(sys, GRAINS_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+// HookeContactForceModel constructor (HookeContactForceModel.cpp:11-20)
+double hooke_beta(double en) { return std::log(en) / std::sqrt(kPi * kPi + std::log(en) * std::log(en)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device allocation
+
+void grains_gpu_system::free_device() {
+  for (void* p : allocations) (void)hipFree(p);
+  allocations.clear();
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+  ev0 = ev1 = nullptr;
+  stream = nullptr;
+}
+
+grains_gpu_system::~grains_gpu_system() {
+  if (device >= 0) (void)hipSetDevice(device);
+  free_device();
+}
+
+template <typename T>
+static hipError_t track(grains_gpu_system* s, T** p, size_t n) {
+  hipError_t e = dalloc(p, n);
+  if (e == hipSuccess) s->allocations.push_back((void*)*p);
+  return e;
+}
+
+int grains_gpu_system::allocate() {
+  const size_t n = (size_t)nparticles;
+  for (int b = 0; b < 2; ++b)
+    for (int a = 0; a < 3; ++a) {
+      HIPCHECK(this, track(this, &P.x[b][a], n));
+      HIPCHECK(this, track(this, &P.v[b][a], n));
+      HIPCHECK(this, track(this, &P.w[b][a], n));
+    }
+  for (int a = 0; a < 3; ++a) {
+    HIPCHECK(this, track(this, &P.wb[a], n));
+    HIPCHECK(this, track(this, &P.F[a], n));
+    HIPCHECK(this, track(this, &P.T[a], n));
+    if (cfg.scheme == GRAINS_ADAMS_BASHFORTH2) {
+      HIPCHECK(this, track(this, &P.v_nm2[a], n));
+      HIPCHECK(this, track(this, &P.dUdt_nm2[a], n));
+      HIPCHECK(this, track(this, &P.wb_nm2[a], n));
+      HIPCHECK(this, track(this, &P.dWdt_nm2[a], n));
+    } else {
+      P.v_nm2[a] = P.dUdt_nm2[a] = P.wb_nm2[a] = P.dWdt_nm2[a] = nullptr;
+    }
+  }
+  for (int a = 0; a < 4; ++a) HIPCHECK(this, track(this, &P.q[a], n));
+  HIPCHECK(this, track(this, &P.rad, n));
+  HIPCHECK(this, track(this, &P.mass, n));
+  HIPCHECK(this, track(this, &P.inertia, n));
+  HIPCHECK(this, track(this, &P.crust, n));
+  HIPCHECK(this, track(this, &P.wfac, n));
+  HIPCHECK(this, track(this, &P.cfac, n));
+  HIPCHECK(this, track(this, &P.req, n));
+  HIPCHECK(this, track(this, &P.id, n));
+  HIPCHECK(this, track(this, &P.mat, n));
+
+  HIPCHECK(this, track(this, &S.key, n));
+  HIPCHECK(this, track(this, &S.slot, n));
+  HIPCHECK(this, track(this, &S.count, (size_t)grid.ncells));
+  HIPCHECK(this, track(this, &S.start, (size_t)grid.ncells + 1));
+  HIPCHECK(this, track(this, &S.block_sums, (size_t)scan_blocks(grid.ncells) + 1));
+  HIPCHECK(this, track(this, &S.tmp_order, n));
+  HIPCHECK(this, track(this, &S.order, n));
+  HIPCHECK(this, track(this, &S.sid, n));
+  HIPCHECK(this, track(this, &S.posr, 4 * n));
+
+  HIPCHECK(this, track(this, &d_obs, obstacles.size()));
+  HIPCHECK(this, track(this, &d_models, (size_t)kMaxMaterials * kMaxMaterials));
+  HIPCHECK(this, track(this, &d_status, 1));
+  pair_cap = (int)std::min<size_t>(16 * n + 4096, (size_t)1 << 28);
+  HIPCHECK(this, track(this, &d_pairs, 2 * (size_t)pair_cap));
+  // index arrays start zeroed so that every stored index is in range
+  HIPCHECK(this, hipMemsetAsync(S.tmp_order, 0, sizeof(int) * n, stream));
+  HIPCHECK(this, hipMemsetAsync(S.order, 0, sizeof(int) * n, stream));
+  HIPCHECK(this, hipMemsetAsync(S.sid, 0, sizeof(int) * n, stream));
+  HIPCHECK(this, hipMemsetAsync(S.posr, 0, sizeof(double) * 4 * n, stream));
+  HIPCHECK(this, hipMemsetAsync(S.start, 0, sizeof(int) * ((size_t)grid.ncells + 1), stream));
+  HIPCHECK(this, hipMemsetAsync(S.count, 0, sizeof(int) * (size_t)grid.ncells, stream));
+  HIPCHECK(this, hipMemsetAsync(d_status, 0, sizeof(DevStatus), stream));
+  return GRAINS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grid: LinkedCell::set (LinkedCell.cpp:47-140) with the cell size of
+// Grains::Construction (Grains.cpp:1010-1022) and defineLinkedCell (:2522-2527)
+
+int grains_gpu_system::build_grid() {
+  double rmax = 0.;
+  for (double r : h_rad) rmax = std::max(rmax, r);
+  if (rmax < 1.e-12) rmax = 1.e16;
+  double factor = cfg.cell_size_factor < 1. ? 1. : cfg.cell_size_factor;
+  double cellsize = 2. * (factor * rmax);
+  long total = 1;
+  for (int a = 0; a < 3; ++a) {
+    double L = cfg.max[a] - cfg.origin[a];
+    if (!(L > 0.)) return fail(this, GRAINS_CONFIG_ERROR, "domain size must be positive on every axis");
+    int n = 1;
+    if (cellsize > kEps) {
+      n = (int)((L + kEps) / cellsize);
+      if (!n) n = 1;
+    }
+    grid.edge[a] = L / n;
+    grid.L[a] = L;
+    grid.periodic[a] = cfg.periodic[a] ? 1 : 0;
+    grid.n[a] = n;
+    grid.lo[a] = cfg.origin[a];
+    grid.off[a] = 0;
+    if (grid.periodic[a]) {
+      if (n < 2)
+        return fail(this, GRAINS_CONFIG_ERROR,
+                    "a periodic direction needs at least 2 cells of size 2*Rmax*CellSizeFactor");
+      grid.lo[a] = grid.lo[a] + (-grid.edge[a]);  // Point3::Move(-edge)
+      grid.off[a] = 1;
+    }
+    total *= n;
+  }
+  if (total > (1L << 30)) return fail(this, GRAINS_CONFIG_ERROR, "too many linked cells");
+  grid.ncells = (int)total;
+  return GRAINS_OK;
+}
+
+// ---------------------------------------------------------------------------
+
+int grains_gpu_system::upload() {
+  const int n = nparticles;
+  std::vector<double> buf((size_t)n);
+  auto up = [&](double* dst, const std::vector<double>& src, int stride, int comp) -> hipError_t {
+    for (int i = 0; i < n; ++i) buf[i] = src[(size_t)stride * i + comp];
+    return hipMemcpy(dst, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+  };
+  for (int a = 0; a < 3; ++a) {
+    HIPCHECK(this, up(P.x[0][a], h_x, 3, a));
+    HIPCHECK(this, up(P.v[0][a], h_v, 3, a));
+    HIPCHECK(this, up(P.w[0][a], h_w, 3, a));
+    HIPCHECK(this, up(P.wb[a], h_wb, 3, a));
+    HIPCHECK(this, hipMemset(P.F[a], 0, sizeof(double) * n));
+    HIPCHECK(this, hipMemset(P.T[a], 0, sizeof(double) * n));
+    if (P.v_nm2[a]) {
+      HIPCHECK(this, hipMemset(P.v_nm2[a], 0, sizeof(double) * n));
+      HIPCHECK(this, hipMemset(P.dUdt_nm2[a], 0, sizeof(double) * n));
+      HIPCHECK(this, hipMemset(P.wb_nm2[a], 0, sizeof(double) * n));
+      HIPCHECK(this, hipMemset(P.dWdt_nm2[a], 0, sizeof(double) * n));
+    }
+  }
+  for (int a = 0; a < 4; ++a) HIPCHECK(this, up(P.q[a], h_q, 4, a));
+  HIPCHECK(this, hipMemcpy(P.rad, h_rad.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(P.mass, h_mass.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(P.inertia, h_inertia.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(P.crust, h_crust.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(P.wfac, h_wfac.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(P.cfac, h_cfac.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(P.req, h_req.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  std::vector<int> ids((size_t)n);
+  for (int i = 0; i < n; ++i) ids[i] = i + 1;
+  HIPCHECK(this, hipMemcpy(P.id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(P.mat, h_mat.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  if (!obstacles.empty())
+    HIPCHECK(this, hipMemcpy(d_obs, obstacles.data(), sizeof(BoxObstacle) * obstacles.size(),
+                             hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(d_models, models, sizeof(models), hipMemcpyHostToDevice));
+  cur = 0;
+  return GRAINS_OK;
+}
+
+StepParams grains_gpu_system::params() const {
+  StepParams sp;
+  for (int a = 0; a < 3; ++a) sp.g[a] = cfg.gravity[a];
+  sp.dt = cfg.dt;
+  sp.n = nparticles;
+  sp.cur = cur;
+  sp.nobs = (int)obstacles.size();
+  sp.nmat = kMaxMaterials;
+  sp.record_pairs = record_pairs;
+  sp.pair_cap = pair_cap;
+  sp.scheme = cfg.scheme;
+  sp.first_step = 0;
+  return sp;
+}
+
+int grains_gpu_system::bin_current() {
+  StepParams sp = params();
+  HIPCHECK(this, launch_key(P, S, grid, sp, d_status, stream));
+  HIPCHECK(this, launch_bin(P, S, grid, nparticles, cur & 1, stream));
+  return GRAINS_OK;
+}
+
+int grains_gpu_system::reset_pair_counter() {
+  if (!record_pairs) return GRAINS_OK;
+  // npairs and pair_overflow live at fixed offsets of DevStatus
+  HIPCHECK(this, launch_clear(&d_status->npairs, 2, stream));
+  return GRAINS_OK;
+}
+
+int grains_gpu_system::mark(std::vector<hipEvent_t>& evs, size_t& n) {
+  if (n >= evs.size()) return fail(this, GRAINS_ARGUMENT_ERROR, "profile event buffer exhausted");
+  HIPCHECK(this, hipEventRecord(evs[n++], stream));
+  return GRAINS_OK;
+}
+
+int grains_gpu_system::bin_launch() {
+  int e;
+  if (prof && (e = mark(prof->bin_ev, prof->nbin))) return e;
+  HIPCHECK(this, launch_bin(P, S, grid, nparticles, cur & 1, stream));
+  if (prof && (e = mark(prof->bin_ev, prof->nbin))) return e;
+  return GRAINS_OK;
+}
+
+int grains_gpu_system::forces(int mode, int first_step) {
+  int e = reset_pair_counter();
+  if (e) return e;
+  StepParams sp = params();
+  sp.first_step = first_step;
+  if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
+  HIPCHECK(this, launch_forces(mode, cfg.precision == GRAINS_MIXED, P, S, grid, d_obs, d_models, sp, d_status,
+                               d_pairs, stream));
+  if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
+  if (mode == kModeForcesKick) cur ^= 2;
+  if (mode == kModeFused) cur ^= 3;
+  return GRAINS_OK;
+}
+
+// After an interrupted launch sequence the per-cell counts may hold stale
+// arrivals: clear them and bin the current positions again.
+int grains_gpu_system::recover() {
+  if (!needs_rebin) return GRAINS_OK;
+  HIPCHECK(this, launch_clear(S.count, grid.ncells, stream));
+  needs_rebin = false;
+  return bin_current();
+}
+
+int grains_gpu_system::enqueue_steps(int nsteps) {
+  int e = recover();
+  if (e) return e;
+  e = enqueue_steps_impl(nsteps);
+  if (e) needs_rebin = true;
+  return e;
+}
+
+// One batch of time steps (Grains::Simulation loop body, Grains.cpp:286-452)
+int grains_gpu_system::enqueue_steps_impl(int nsteps) {
+  if (nsteps <= 0) return GRAINS_OK;
+  int e;
+  if (cfg.scheme == GRAINS_LEAPFROG) {
+    // Move(dt/2, dt) -> LinkUpdate -> forces -> advanceVelocity(dt/2); the
+    // forces, the second half kick and the next Move share one kernel.
+    StepParams sp = params();
+    HIPCHECK(this, launch_move(P, S, grid, sp, d_status, 0.5 * cfg.dt, cfg.dt, stream));
+    if ((e = bin_launch())) return e;
+    for (int i = 1; i < nsteps; ++i) {
+      if ((e = forces(kModeFused, 0))) return e;
+      if ((e = bin_launch())) return e;
+    }
+    if ((e = forces(kModeForcesKick, 0))) return e;
+  } else {
+    // forces -> Move(dt, dt) -> LinkUpdate (Grains.cpp:365-377)
+    for (int i = 0; i < nsteps; ++i) {
+      int first = (cfg.scheme == GRAINS_ADAMS_BASHFORTH2 && steps_done + i == 0) ? 1 : 0;
+      if ((e = forces(kModeFused, first))) return e;
+      if ((e = bin_launch())) return e;
+    }
+  }
+  steps_done += nsteps;
+  for (int i = 0; i < nsteps; ++i) time += cfg.dt;
+  return GRAINS_OK;
+}
+
+int grains_gpu_system::check_status() {
+  DevStatus st;
+  HIPCHECK(this, hipMemcpyAsync(&st, d_status, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(this, hipStreamSynchronize(stream));
+  last_status = st;
+  if (st.code) {
+    static const char* names[] = {"", "ContactError", "MotionError", "SimulationError"};
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%s raised on the device (bodies %d and %d)", names[st.code & 3], st.id0, st.id1);
+    return fail(this, st.code, buf);
+  }
+  if (st.pair_overflow) return fail(this, GRAINS_DEVICE_ERROR, "contact record buffer overflow");
+  return GRAINS_OK;
+}
+
+int grains_gpu_system::initialize() {
+  if (initialized) return fail(this, GRAINS_ARGUMENT_ERROR, "system already initialised");
+  if (cfg.scheme < 0 || cfg.scheme > 3) return fail(this, GRAINS_CONFIG_ERROR, "unknown time integrator");
+  if (!(cfg.dt > 0.)) return fail(this, GRAINS_CONFIG_ERROR, "time step must be positive");
+  // ContactBuilderFactory::checkContactForceModelsExist (ContactBuilderFactory.cpp:31-76):
+  // every material pair involving a particle material needs a model
+  std::vector<int> used(kMaxMaterials, 0);
+  for (int m : h_mat) used[m] |= 1;
+  for (auto& o : obstacles) used[o.mat] |= 2;
+  for (int a = 0; a < kMaxMaterials; ++a)
+    for (int b = 0; b < kMaxMaterials; ++b)
+      if ((used[a] & 1) && used[b] && !models[a * kMaxMaterials + b].defined) {
+        if (has_default) models[a * kMaxMaterials + b] = default_model;
+        else {
+          char buf[128];
+          snprintf(buf, sizeof(buf), "no contact force model for materials %d and %d", a, b);
+          return fail(this, GRAINS_CONFIG_ERROR, buf);
+        }
+      }
+  int e = build_grid();
+  if (e) return e;
+  HIPCHECK(this, hipSetDevice(device));
+  HIPCHECK(this, hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIPCHECK(this, hipEventCreate(&ev0));
+  HIPCHECK(this, hipEventCreate(&ev1));
+  if ((e = allocate())) return e;
+  if ((e = upload())) return e;
+  if ((e = bin_current())) return e;
+  initialized = true;
+  // LeapFrog: accelerations at the initial time (Grains.cpp:169-174)
+  if (cfg.scheme == GRAINS_LEAPFROG) {
+    if ((e = forces(kModeForcesOnly, 0))) return e;
+  }
+  return check_status();
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int grains_gpu_create(const grains_gpu_config* cfg, grains_gpu_system** out) {
+  if (!cfg || !out) return GRAINS_ARGUMENT_ERROR;
+  grains_gpu_system* s = new grains_gpu_system();
+  s->cfg = *cfg;
+  s->device = cfg->device;
+  s->time = cfg->t_start;
+  *out = s;
+  return GRAINS_OK;
+}
+
+void grains_gpu_destroy(grains_gpu_system* sys) { delete sys; }
+
+int grains_gpu_set_contact_model(grains_gpu_system* s, int matA, int matB, double kn, double en, double etat,
+                                 double muc, double kr) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  if (matA >= kMaxMaterials || matB >= kMaxMaterials) return fail(s, GRAINS_ARGUMENT_ERROR, "material index too large");
+  if (!(en > 0.) || en > 1.) return fail(s, GRAINS_CONFIG_ERROR, "en must lie in (0, 1]");
+  HookeModel m;
+  m.kn = kn; m.en = en; m.etat = etat; m.muc = muc; m.kr = kr;
+  m.beta = hooke_beta(en);
+  m.defined = 1;
+  m.pad = 0;
+  if (matA < 0 || matB < 0) {
+    s->default_model = m;
+    s->has_default = true;
+  } else {
+    s->models[matA * kMaxMaterials + matB] = m;
+    s->models[matB * kMaxMaterials + matA] = m;
+  }
+  if (s->initialized)
+    HIPCHECK(s, hipMemcpy(s->d_models, s->models, sizeof(s->models), hipMemcpyHostToDevice));
+  return GRAINS_OK;
+}
+
+int grains_gpu_add_spheres(grains_gpu_system* s, size_t n, const double* radius, const double* density,
+                           const double* crust, const int* material, const double* x, const double* v,
+                           const double* w) {
+  if (!s || !radius || !density || !crust || !material || !x) return GRAINS_ARGUMENT_ERROR;
+  if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "particles must be added before initialisation");
+  for (size_t i = 0; i < n; ++i) {
+    double r = radius[i], rho = density[i];
+    if (!(r > 0.) || !(rho > 0.) || !(crust[i] > 0.))
+      return fail(s, GRAINS_CONFIG_ERROR, "radius, density and crust thickness must be positive");
+    if (material[i] < 0 || material[i] >= kMaxMaterials) return fail(s, GRAINS_ARGUMENT_ERROR, "bad material");
+    // Particle(DOMNode*) mass and inertia (Particle.cpp:157-162, Sphere.cpp:58-64, 152-155)
+    double volume = 4.0 * kPi * std::pow(r, 3.) / 3.0;
+    double mass = rho * volume;
+    double I = 8.0 * kPi * std::pow(r, 5.) / 15.0;
+    I *= rho;
+    double buoy = 1. - s->cfg.fluid_density / rho;
+    s->h_rad.push_back(r);
+    s->h_mass.push_back(mass);
+    s->h_inertia.push_back(I);
+    s->h_crust.push_back(crust[i]);
+    s->h_wfac.push_back(mass * buoy);  // Particle::computeWeight (Particle.cpp:413-418)
+    double cf = 1.;
+    if (s->cfg.fluid_density != 0.) cf -= s->cfg.fluid_density / rho;  // ParticleKinematics.cpp:485-487
+    s->h_cfac.push_back(cf);
+    s->h_req.push_back(std::pow((0.75 / kPi) * volume, 1. / 3.));  // Component.cpp:205-208
+    s->h_mat.push_back(material[i]);
+    d3 wv{0., 0., 0.};
+    for (int a = 0; a < 3; ++a) {
+      s->h_x.push_back(x[3 * i + a]);
+      s->h_v.push_back(v ? v[3 * i + a] : 0.);
+      s->h_w.push_back(w ? w[3 * i + a] : 0.);
+    }
+    if (w) wv = d3{w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+    quat q{d3{0., 0., 0.}, 1.};
+    d3 wb = to_body(q, wv);  // ParticleKinematics::setAngularVelocity
+    s->h_wb.push_back(wb.x);
+    s->h_wb.push_back(wb.y);
+    s->h_wb.push_back(wb.z);
+    s->h_q.push_back(0.);
+    s->h_q.push_back(0.);
+    s->h_q.push_back(0.);
+    s->h_q.push_back(1.);
+  }
+  s->nparticles = (int)s->h_rad.size();
+  return GRAINS_OK;
+}
+
+int grains_gpu_add_box(grains_gpu_system* s, const double* L, const double* centre, const double* basis, double crust,
+                       int material) {
+  if (!s || !L || !centre) return GRAINS_ARGUMENT_ERROR;
+  if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "obstacles must be added before initialisation");
+  if (material < 0 || material >= kMaxMaterials) return fail(s, GRAINS_ARGUMENT_ERROR, "bad material");
+  BoxObstacle o;
+  std::memset(&o, 0, sizeof(o));
+  o.id = -(int)s->obstacles.size() - 1;
+  double e[3] = {std::fabs(L[0]) / 2., std::fabs(L[1]) / 2., std::fabs(L[2]) / 2.};
+  for (int a = 0; a < 3; ++a) {
+    o.extent[a] = e[a];
+    o.centre[a] = centre[a];
+  }
+  for (int i = 0; i < 9; ++i) o.B[i] = basis ? basis[i] : ((i % 4) == 0 ? 1. : 0.);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o.Binv[3 * i + j] = o.B[3 * j + i];
+  d3 c{centre[0], centre[1], centre[2]};
+  // Transform::setToInverseTransform (Transform.cpp:71-77)
+  for (int i = 0; i < 3; ++i) o.invorigin[i] = dot(-d3{o.Binv[3 * i], o.Binv[3 * i + 1], o.Binv[3 * i + 2]}, c);
+  // Box::setCornersFaces (Box.cpp:92-110)
+  const double cs[8][3] = {{-e[0], -e[1], -e[2]}, {e[0], -e[1], -e[2]}, {e[0], -e[1], e[2]}, {-e[0], -e[1], e[2]},
+                           {-e[0], e[1], -e[2]},  {e[0], e[1], -e[2]},  {e[0], e[1], e[2]},  {-e[0], e[1], e[2]}};
+  for (int k = 0; k < 8; ++k)
+    for (int a = 0; a < 3; ++a) o.corners[3 * k + a] = cs[k][a];
+  o.crust = crust;
+  o.mat = material;
+  o.req = std::pow((0.75 / kPi) * (8.0 * e[0] * e[1] * e[2]), 1. / 3.);
+  // Convex::bbox with Box::support (Convex.cpp:29-41, Box.cpp:307-316), BBox::setValue (BBox.cpp:235-239)
+  d3 ext{e[0], e[1], e[2]};
+  auto support = [&](d3 d) {
+    if (norm(d) < kEps) return d3{0., 0., 0.};
+    return d3{d.x < 0. ? -ext.x : ext.x, d.y < 0. ? -ext.y : ext.y, d.z < 0. ? -ext.z : ext.z};
+  };
+  double mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    d3 row{o.B[3 * a], o.B[3 * a + 1], o.B[3 * a + 2]};
+    mn[a] = centre[a] + dot(row, support(-row));
+    mx[a] = centre[a] + dot(row, support(row));
+  }
+  for (int a = 0; a < 3; ++a) {
+    o.aabb_e[a] = (mx[a] - mn[a]) / 2.;
+    o.aabb_c[a] = mn[a] + o.aabb_e[a];
+  }
+  s->obstacles.push_back(o);
+  return GRAINS_OK;
+}
+
+int grains_gpu_initialize(grains_gpu_system* s) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  return s->initialize();
+}
+
+int grains_gpu_step(grains_gpu_system* s, int nsteps) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  if (!s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "system not initialised");
+  HIPCHECK(s, hipSetDevice(s->device));
+  int e = s->enqueue_steps(nsteps);
+  if (e) return e;
+  return s->check_status();
+}
+
+int grains_gpu_step_timed(grains_gpu_system* s, int nsteps, double* ms) {
+  if (!s || !ms) return GRAINS_ARGUMENT_ERROR;
+  if (!s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "system not initialised");
+  HIPCHECK(s, hipSetDevice(s->device));
+  HIPCHECK(s, hipEventRecord(s->ev0, s->stream));
+  int e = s->enqueue_steps(nsteps);
+  if (e) return e;
+  HIPCHECK(s, hipEventRecord(s->ev1, s->stream));
+  HIPCHECK(s, hipEventSynchronize(s->ev1));
+  float t = 0.f;
+  HIPCHECK(s, hipEventElapsedTime(&t, s->ev0, s->ev1));
+  *ms = t;
+  return s->check_status();
+}
+
+int grains_gpu_profile_steps(grains_gpu_system* s, int nsteps, double* out) {
+  if (!s || !out || nsteps <= 0 || nsteps > 100000) return GRAINS_ARGUMENT_ERROR;
+  if (!s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "system not initialised");
+  HIPCHECK(s, hipSetDevice(s->device));
+  grains_gpu_system::Profile pr;
+  pr.force_ev.resize(2 * (size_t)nsteps + 2);
+  pr.bin_ev.resize(2 * (size_t)nsteps + 2);
+  int e = GRAINS_OK;
+  for (auto& ev : pr.force_ev) HIPCHECK(s, hipEventCreate(&ev));
+  for (auto& ev : pr.bin_ev) HIPCHECK(s, hipEventCreate(&ev));
+  HIPCHECK(s, hipEventRecord(s->ev0, s->stream));
+  s->prof = &pr;
+  e = s->enqueue_steps(nsteps);
+  s->prof = nullptr;
+  if (!e) {
+    HIPCHECK(s, hipEventRecord(s->ev1, s->stream));
+    HIPCHECK(s, hipEventSynchronize(s->ev1));
+    float t = 0.f;
+    HIPCHECK(s, hipEventElapsedTime(&t, s->ev0, s->ev1));
+    out[0] = t;
+    double fsum = 0., bsum = 0.;
+    for (size_t i = 0; i + 1 < pr.nforce; i += 2) {
+      HIPCHECK(s, hipEventElapsedTime(&t, pr.force_ev[i], pr.force_ev[i + 1]));
+      fsum += t;
+    }
+    for (size_t i = 0; i + 1 < pr.nbin; i += 2) {
+      HIPCHECK(s, hipEventElapsedTime(&t, pr.bin_ev[i], pr.bin_ev[i + 1]));
+      bsum += t;
+    }
+    out[1] = fsum;
+    out[2] = (double)(pr.nforce / 2);
+    out[3] = bsum;
+    out[4] = (double)(pr.nbin / 2);
+  }
+  for (auto& ev : pr.force_ev) (void)hipEventDestroy(ev);
+  for (auto& ev : pr.bin_ev) (void)hipEventDestroy(ev);
+  if (e) return e;
+  return s->check_status();
+}
+
+int grains_gpu_synchronize(grains_gpu_system* s) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  return s->check_status();
+}
+
+int grains_gpu_compute_forces(grains_gpu_system* s) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  HIPCHECK(s, hipSetDevice(s->device));
+  int e = s->forces(kModeForcesOnly, 0);
+  if (e) return e;
+  return s->check_status();
+}
+
+int grains_gpu_get_state(grains_gpu_system* s, double* x, double* v, double* w, double* wb, double* q, double* F,
+                         double* T) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  HIPCHECK(s, hipSetDevice(s->device));
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  const int n = s->nparticles;
+  std::vector<int> ids((size_t)n);
+  HIPCHECK(s, hipMemcpy(ids.data(), s->P.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<double> buf((size_t)n);
+  const int cx = s->cur & 1, cv = (s->cur >> 1) & 1;
+  auto down = [&](double* dst, double* src, int stride, int comp) -> hipError_t {
+    if (!dst) return hipSuccess;
+    hipError_t e = hipMemcpy(buf.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < n; ++i) dst[(size_t)stride * (ids[i] - 1) + comp] = buf[i];
+    return hipSuccess;
+  };
+  for (int a = 0; a < 3; ++a) {
+    HIPCHECK(s, down(x, s->P.x[cx][a], 3, a));
+    HIPCHECK(s, down(v, s->P.v[cv][a], 3, a));
+    HIPCHECK(s, down(w, s->P.w[cv][a], 3, a));
+    HIPCHECK(s, down(wb, s->P.wb[a], 3, a));
+    HIPCHECK(s, down(F, s->P.F[a], 3, a));
+    HIPCHECK(s, down(T, s->P.T[a], 3, a));
+  }
+  for (int a = 0; a < 4; ++a) HIPCHECK(s, down(q, s->P.q[a], 4, a));
+  return s->check_status();
+}
+
+int grains_gpu_set_state(grains_gpu_system* s, const double* x, const double* v, const double* w, const double* q,
+                         int recompute_forces) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  HIPCHECK(s, hipSetDevice(s->device));
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  const int n = s->nparticles;
+  std::vector<int> ids((size_t)n);
+  HIPCHECK(s, hipMemcpy(ids.data(), s->P.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  const int cx = s->cur & 1, cv = (s->cur >> 1) & 1;
+  std::vector<double> buf((size_t)n);
+  auto get = [&](double* src, std::vector<double>& dst, int stride, int comp) -> hipError_t {
+    hipError_t e = hipMemcpy(buf.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost);
+    for (int i = 0; i < n; ++i) dst[(size_t)stride * (ids[i] - 1) + comp] = buf[i];
+    return e;
+  };
+  auto put = [&](double* dst, const std::vector<double>& src, int stride, int comp) -> hipError_t {
+    for (int i = 0; i < n; ++i) buf[i] = src[(size_t)stride * (ids[i] - 1) + comp];
+    return hipMemcpy(dst, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+  };
+  // current q and w (needed to rebuild the body-fixed angular velocity)
+  std::vector<double> qq((size_t)4 * n), ww((size_t)3 * n);
+  for (int a = 0; a < 4; ++a) HIPCHECK(s, get(s->P.q[a], qq, 4, a));
+  for (int a = 0; a < 3; ++a) HIPCHECK(s, get(s->P.w[cv][a], ww, 3, a));
+  if (q) std::copy(q, q + 4 * (size_t)n, qq.begin());
+  if (w) std::copy(w, w + 3 * (size_t)n, ww.begin());
+  std::vector<double> wbv((size_t)3 * n);
+  for (int i = 0; i < n; ++i) {
+    quat qi{d3{qq[4 * i], qq[4 * i + 1], qq[4 * i + 2]}, qq[4 * i + 3]};
+    d3 wb = to_body(qi, d3{ww[3 * i], ww[3 * i + 1], ww[3 * i + 2]});
+    wbv[3 * i] = wb.x;
+    wbv[3 * i + 1] = wb.y;
+    wbv[3 * i + 2] = wb.z;
+  }
+  std::vector<double> tmp;
+  for (int a = 0; a < 3; ++a) {
+    if (x) {
+      tmp.assign(x, x + 3 * (size_t)n);
+      HIPCHECK(s, put(s->P.x[cx][a], tmp, 3, a));
+    }
+    if (v) {
+      tmp.assign(v, v + 3 * (size_t)n);
+      HIPCHECK(s, put(s->P.v[cv][a], tmp, 3, a));
+    }
+    HIPCHECK(s, put(s->P.w[cv][a], ww, 3, a));
+    HIPCHECK(s, put(s->P.wb[a], wbv, 3, a));
+  }
+  for (int a = 0; a < 4; ++a) HIPCHECK(s, put(s->P.q[a], qq, 4, a));
+  int e = s->bin_current();
+  if (e) return e;
+  if (recompute_forces) {
+    if ((e = s->forces(kModeForcesOnly, 0))) return e;
+  }
+  return s->check_status();
+}
+
+int grains_gpu_get_masses(grains_gpu_system* s, double* mass, double* inertia, double* weight) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  for (int i = 0; i < s->nparticles; ++i) {
+    if (mass) mass[i] = s->h_mass[i];
+    if (inertia) inertia[i] = s->h_inertia[i];
+    if (weight)
+      for (int a = 0; a < 3; ++a) weight[3 * i + a] = s->cfg.gravity[a] * s->h_wfac[i];
+  }
+  return GRAINS_OK;
+}
+
+int grains_gpu_set_record_pairs(grains_gpu_system* s, int on) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  s->record_pairs = on ? 1 : 0;
+  return GRAINS_OK;
+}
+
+int grains_gpu_get_pairs(grains_gpu_system* s, int* buf, size_t cap, size_t* n) {
+  if (!s || !s->initialized || !n) return GRAINS_ARGUMENT_ERROR;
+  int e = s->check_status();
+  if (e) return e;
+  size_t np = (size_t)std::min(s->last_status.npairs, s->pair_cap);
+  *n = np;
+  if (buf && cap) {
+    size_t m = std::min(np, cap);
+    HIPCHECK(s, hipMemcpy(buf, s->d_pairs, sizeof(int) * 2 * m, hipMemcpyDeviceToHost));
+  }
+  return GRAINS_OK;
+}
+
+size_t grains_gpu_num_particles(const grains_gpu_system* s) { return s ? (size_t)s->nparticles : 0; }
+double grains_gpu_time(const grains_gpu_system* s) { return s ? s->time : 0.; }
+void grains_gpu_set_time(grains_gpu_system* s, double t) {
+  if (s) s->time = t;
+}
+
+int grains_gpu_grid(const grains_gpu_system* s, int* dims, double* edge) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  for (int a = 0; a < 3; ++a) {
+    if (dims) dims[a] = s->grid.n[a];
+    if (edge) edge[a] = s->grid.edge[a];
+  }
+  return GRAINS_OK;
+}
+
+int grains_gpu_error_ids(grains_gpu_system* s, int* id0, int* id1) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  if (id0) *id0 = s->last_status.id0;
+  if (id1) *id1 = s->last_status.id1;
+  return GRAINS_OK;
+}
+
+int grains_gpu_recoveries(grains_gpu_system* s, long* n) {
+  if (!s || !n) return GRAINS_ARGUMENT_ERROR;
+  *n = s->last_status.n_recovered;
+  return GRAINS_OK;
+}
+
+const char* grains_gpu_last_error(const grains_gpu_system* s) { return s ? s->last_error.c_str() : "null system"; }
+
+}  // extern "C"

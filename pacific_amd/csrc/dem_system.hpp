@@ -1,0 +1,68 @@
+// The system object behind the opaque grains_gpu_system handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/grains_gpu.h"
+#include "dem_types.hpp"
+
+struct grains_gpu_system {
+  grains_gpu_config cfg{};
+  int device = 0;
+  int nparticles = 0;
+  double time = 0.;
+  long steps_done = 0;
+  bool initialized = false;
+  int cur = 0;  // bit 0: position buffer, bit 1: velocity buffer
+  int record_pairs = 0;
+  bool needs_rebin = false;  // a launch sequence was interrupted
+  int pair_cap = 0;
+  std::string last_error;
+  dem::DevStatus last_status{};
+
+  // host copies of the construction data, id order
+  std::vector<double> h_rad, h_mass, h_inertia, h_crust, h_wfac, h_cfac, h_req;
+  std::vector<double> h_x, h_v, h_w, h_wb, h_q;
+  std::vector<int> h_mat;
+  std::vector<dem::BoxObstacle> obstacles;
+  dem::HookeModel models[dem::kMaxMaterials * dem::kMaxMaterials]{};
+  dem::HookeModel default_model{};
+  bool has_default = false;
+
+  // device
+  dem::Grid grid{};
+  dem::Particles P{};
+  dem::Sorted S{};
+  dem::BoxObstacle* d_obs = nullptr;
+  dem::HookeModel* d_models = nullptr;
+  dem::DevStatus* d_status = nullptr;
+  int* d_pairs = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<void*> allocations;
+
+  ~grains_gpu_system();
+  int initialize();
+  int build_grid();
+  int allocate();
+  int upload();
+  void free_device();
+  dem::StepParams params() const;
+  int bin_current();
+  int reset_pair_counter();
+  int forces(int mode, int first_step);
+  // optional per-launch HIP events (force sweeps and binning sequences)
+  struct Profile {
+    std::vector<hipEvent_t> force_ev, bin_ev;
+    size_t nforce = 0, nbin = 0;
+  };
+  Profile* prof = nullptr;
+  int mark(std::vector<hipEvent_t>& evs, size_t& n);
+  int bin_launch();
+  int enqueue_steps(int nsteps);
+  int enqueue_steps_impl(int nsteps);
+  int recover();
+  int check_status();
+};

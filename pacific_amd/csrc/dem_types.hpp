@@ -1,0 +1,113 @@
+// Device-side data layout of one DEM system (structure of arrays in HBM) and
+// the parameter blocks the kernels take by value.
+#pragma once
+#include <cstdint>
+
+namespace dem {
+
+constexpr int kMaxMaterials = 16;
+
+// Hooke-dashpot-Coulomb parameters of one material pair
+// (HookeContactForceModel.cpp:11-20, 170-199); beta is computed on the host
+// with the host libm so that both sides use the same bits.
+struct HookeModel {
+  double kn, en, etat, muc, kr, beta;
+  int defined;
+  int pad;
+};
+
+// Static box obstacle (Box.cpp:27-125, Transform.cpp:71-77, Convex.cpp:29-41)
+struct BoxObstacle {
+  double B[9];        // basis rows (world <- box)
+  double Binv[9];     // transpose (box <- world)
+  double centre[3];
+  double invorigin[3];
+  double extent[3];
+  double corners[24];  // Box::setCornersFaces numbering
+  double aabb_c[3], aabb_e[3];
+  double crust, req;
+  int mat, id;
+};
+
+// Linked-cell grid.  The GPU grid holds the reference's interior cells; on a
+// periodic axis the reference adds one ghost layer on each side, so
+// GPU index = reference index - off[a] with off[a] = 1.
+struct Grid {
+  double lo[3];     // reference local origin (LinkedCell m_LC_local_origin)
+  double edge[3];   // cell edge per axis
+  double L[3];      // domain size (periodic vector length)
+  int n[3];         // GPU cells per axis
+  int off[3];       // 1 on periodic axes, 0 otherwise
+  int periodic[3];
+  int ncells;
+  int pad;
+};
+
+// Particle state.  x, v, w are double buffered (index cur / 1-cur) because the
+// force sweep reads neighbours' positions and velocities while a fused
+// kernel writes the next ones.
+struct Particles {
+  double* x[2][3];
+  double* v[2][3];
+  double* w[2][3];   // space-fixed angular velocity
+  double* wb[3];     // body-fixed angular velocity (integrated state)
+  double* q[4];      // quaternion x, y, z, w
+  double* F[3];
+  double* T[3];
+  double* rad;
+  double* mass;
+  double* inertia;   // sphere: I[0] = I[3] = I[5]
+  double* crust;
+  double* wfac;      // mass * (1 - rho_f / rho): weight = wfac * g
+  double* cfac;      // coupling factor 1 - rho_f / rho
+  double* req;       // equivalent-sphere radius (rolling resistance)
+  int* id;
+  int* mat;
+  // Adams-Bashforth n-2 state (allocated only for that scheme)
+  double* v_nm2[3];
+  double* dUdt_nm2[3];
+  double* wb_nm2[3];
+  double* dWdt_nm2[3];
+};
+
+// Cell-sorted view rebuilt every step by the binning kernels.
+struct Sorted {
+  int* key;        // [N] cell key of particle (storage index)
+  int* slot;       // [N] arrival slot within the cell (atomic counter)
+  int* count;      // [ncells] particles per cell (zeroed after use)
+  int* start;      // [ncells + 1] exclusive prefix of count
+  int* block_sums; // scan scratch
+  int* tmp_order;  // [N] storage index by (cell, slot)
+  int* order;      // [N] storage index by (cell, id)
+  int* sid;        // [N] particle id by sorted slot
+  double* posr;    // [4N] x, y, z, radius by sorted slot
+};
+
+// Device-side error / statistics block (one per system)
+struct DevStatus {
+  int code;          // first error: 1 contact, 2 motion, 3 simulation
+  int id0, id1;
+  int npairs;        // directed contact records written
+  int pair_overflow;
+  int n_recovered;   // ContactError recoveries
+  int pad[2];
+  double overlap_max;
+};
+
+enum ErrCode { kOk = 0, kContactError = 1, kMotionError = 2, kSimulationError = 3 };
+enum Scheme { kLeapFrog = 0, kFirstOrderExplicit = 1, kSecondOrderExplicit = 2, kAdamsBashforth2 = 3 };
+
+struct StepParams {
+  double g[3];
+  double dt;
+  int n;              // number of particles
+  int cur;            // current x/v/w buffer
+  int nobs;
+  int nmat;
+  int record_pairs;
+  int pair_cap;
+  int scheme;
+  int first_step;     // Adams-Bashforth first step (Euler start)
+};
+
+}  // namespace dem

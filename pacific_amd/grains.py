@@ -1,0 +1,253 @@
+"""Python binding of libgrains_gpu.so (the C ABI in include/grains_gpu.h).
+
+Mirrors the Grains3D objects that set up a DEM system -- domain, contact
+models, spheres, box obstacles, time integration -- and exposes the state as
+NumPy arrays in particle-id order.  The binding is ctypes over the C ABI: no
+torch types cross the boundary, and there is no CPU fallback -- if the HIP
+library is missing or no GPU is present, construction raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgrains_gpu.so")
+
+LEAPFROG, FIRST_ORDER_EXPLICIT, SECOND_ORDER_EXPLICIT, ADAMS_BASHFORTH2 = 0, 1, 2, 3
+FP64, MIXED = 0, 1
+SCHEMES = {"SecondOrderLeapFrog": LEAPFROG, "FirstOrderExplicit": FIRST_ORDER_EXPLICIT,
+           "SecondOrderExplicit": SECOND_ORDER_EXPLICIT, "SecondOrderAdamsBashforth": ADAMS_BASHFORTH2}
+ERRORS = {0: "OK", 1: "ContactError", 2: "MotionError", 3: "SimulationError", 4: "ConfigError",
+          5: "DeviceError", 6: "ArgumentError"}
+
+_lib = None
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_double * 3), ("max", ctypes.c_double * 3), ("periodic", ctypes.c_int * 3),
+                ("cell_size_factor", ctypes.c_double), ("gravity", ctypes.c_double * 3),
+                ("fluid_density", ctypes.c_double), ("dt", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("scheme", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+EXPORTS = [
+    "grains_gpu_create", "grains_gpu_create_from_xml", "grains_gpu_destroy", "grains_gpu_set_contact_model",
+    "grains_gpu_add_spheres", "grains_gpu_add_box", "grains_gpu_initialize", "grains_gpu_step",
+    "grains_gpu_step_timed", "grains_gpu_profile_steps", "grains_gpu_synchronize", "grains_gpu_compute_forces", "grains_gpu_get_state",
+    "grains_gpu_set_state", "grains_gpu_get_masses", "grains_gpu_set_record_pairs", "grains_gpu_get_pairs",
+    "grains_gpu_num_particles", "grains_gpu_time", "grains_gpu_set_time", "grains_gpu_grid",
+    "grains_gpu_error_ids", "grains_gpu_recoveries", "grains_gpu_last_error",
+]
+
+
+def lib():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with `python -m pacific_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+        ip, sz = ctypes.POINTER(ctypes.c_int), ctypes.c_size_t
+        L.grains_gpu_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+        if hasattr(L, "grains_gpu_create_from_xml"):
+            L.grains_gpu_create_from_xml.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp)]
+        L.grains_gpu_destroy.argtypes = [vp]
+        L.grains_gpu_destroy.restype = None
+        L.grains_gpu_set_contact_model.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 5
+        L.grains_gpu_add_spheres.argtypes = [vp, sz, dp, dp, dp, ip, dp, dp, dp]
+        L.grains_gpu_add_box.argtypes = [vp, dp, dp, dp, ctypes.c_double, ctypes.c_int]
+        L.grains_gpu_initialize.argtypes = [vp]
+        L.grains_gpu_step.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_step_timed.argtypes = [vp, ctypes.c_int, dp]
+        L.grains_gpu_profile_steps.argtypes = [vp, ctypes.c_int, dp]
+        L.grains_gpu_synchronize.argtypes = [vp]
+        L.grains_gpu_compute_forces.argtypes = [vp]
+        L.grains_gpu_get_state.argtypes = [vp] + [dp] * 7
+        L.grains_gpu_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
+        L.grains_gpu_get_masses.argtypes = [vp, dp, dp, dp]
+        L.grains_gpu_set_record_pairs.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
+        L.grains_gpu_num_particles.argtypes = [vp]
+        L.grains_gpu_num_particles.restype = sz
+        L.grains_gpu_time.argtypes = [vp]
+        L.grains_gpu_time.restype = ctypes.c_double
+        L.grains_gpu_set_time.argtypes = [vp, ctypes.c_double]
+        L.grains_gpu_set_time.restype = None
+        L.grains_gpu_grid.argtypes = [vp, ip, dp]
+        L.grains_gpu_error_ids.argtypes = [vp, ip, ip]
+        L.grains_gpu_recoveries.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
+        L.grains_gpu_last_error.argtypes = [vp]
+        L.grains_gpu_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c64(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+class GrainsError(RuntimeError):
+    def __init__(self, code, msg, ids=(0, 0)):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+        self.kind = ERRORS.get(code, str(code))
+        self.ids = ids
+
+
+class GrainsGPU:
+    """One DEM system on one GPU (spheres + static box obstacles)."""
+
+    def __init__(self, origin, dmax, periodic=(0, 0, 0), cellsize_factor=1.0, gravity=(0.0, 0.0, -9.81),
+                 fluid_density=0.0, dt=1e-5, scheme=LEAPFROG, precision=FP64, t_start=0.0, device=0, _handle=None):
+        L = lib()
+        self.h = ctypes.c_void_p()
+        self.n = 0
+        self.nobs = 0
+        if _handle is not None:
+            self.h = _handle
+            self.n = int(L.grains_gpu_num_particles(self.h))
+            return
+        cfg = Config()
+        for a in range(3):
+            cfg.origin[a] = origin[a]
+            cfg.max[a] = dmax[a]
+            cfg.periodic[a] = int(periodic[a])
+            cfg.gravity[a] = gravity[a]
+        cfg.cell_size_factor = cellsize_factor
+        cfg.fluid_density = fluid_density
+        cfg.dt = dt
+        cfg.t_start = t_start
+        cfg.scheme = scheme
+        cfg.precision = precision
+        cfg.device = device
+        self._check(L.grains_gpu_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    @classmethod
+    def from_xml(cls, path, precision=FP64):
+        h = ctypes.c_void_p()
+        code = lib().grains_gpu_create_from_xml(os.fsencode(path), precision, ctypes.byref(h))
+        if code:
+            msg = lib().grains_gpu_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                lib().grains_gpu_destroy(h)
+            raise GrainsError(code, msg)
+        return cls(None, None, _handle=h)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value and _lib is not None:
+            _lib.grains_gpu_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, code):
+        if code:
+            L = lib()
+            msg = L.grains_gpu_last_error(self.h).decode() if self.h.value else ""
+            a, b = ctypes.c_int(), ctypes.c_int()
+            if self.h.value:
+                L.grains_gpu_error_ids(self.h, ctypes.byref(a), ctypes.byref(b))
+            raise GrainsError(code, msg, (a.value, b.value))
+
+    def contact_model(self, matA, matB, kn, en, muc, kr=0.0, etat=-1.0):
+        self._check(lib().grains_gpu_set_contact_model(self.h, matA, matB, kn, en, etat, muc, kr))
+
+    def add_spheres(self, radius, density, crust, material, x, v=None, w=None):
+        x = _c64(x, (-1, 3))
+        n = len(x)
+        rad = _c64(np.broadcast_to(radius, (n,)))
+        rho = _c64(np.broadcast_to(density, (n,)))
+        cr = _c64(np.broadcast_to(crust, (n,)))
+        mat = np.ascontiguousarray(np.broadcast_to(material, (n,)), dtype=np.int32)
+        v = _c64(v, (-1, 3))
+        w = _c64(w, (-1, 3))
+        self._check(lib().grains_gpu_add_spheres(self.h, n, _d(rad), _d(rho), _d(cr),
+                                                 mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(x), _d(v),
+                                                 _d(w)))
+        self.n += n
+
+    def add_box(self, L, centre, basis=None, crust=5e-5, material=0):
+        Lv, c = _c64(L), _c64(centre)
+        b = _c64(basis, (9,))
+        self._check(lib().grains_gpu_add_box(self.h, _d(Lv), _d(c), _d(b), crust, material))
+        self.nobs += 1
+
+    def init(self):
+        self._check(lib().grains_gpu_initialize(self.h))
+
+    def step(self, n=1):
+        self._check(lib().grains_gpu_step(self.h, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        self._check(lib().grains_gpu_step_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def profile_steps(self, n):
+        """Runs n steps with HIP events around each kernel group; returns
+        dict(total_ms, force_ms, n_force, bin_ms, n_bin)."""
+        out = np.zeros(5)
+        self._check(lib().grains_gpu_profile_steps(self.h, int(n), _d(out)))
+        return {"total_ms": out[0], "force_ms": out[1], "n_force": int(out[2]), "bin_ms": out[3],
+                "n_bin": int(out[4])}
+
+    def synchronize(self):
+        self._check(lib().grains_gpu_synchronize(self.h))
+
+    def compute_forces(self):
+        self._check(lib().grains_gpu_compute_forces(self.h))
+
+    @property
+    def time(self):
+        return lib().grains_gpu_time(self.h)
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "w", "wb", "F", "T")}
+        out["q"] = np.zeros((n, 4))
+        self._check(lib().grains_gpu_get_state(self.h, *[_d(out[k]) for k in ("x", "v", "w", "wb", "q", "F", "T")]))
+        return out
+
+    def set_state(self, x=None, v=None, w=None, q=None, recompute=True):
+        keep = [_c64(a) for a in (x, v, w, q)]
+        self._check(lib().grains_gpu_set_state(self.h, *[_d(a) for a in keep], 1 if recompute else 0))
+
+    def masses(self):
+        n = self.n
+        m, I, wgt = np.zeros(n), np.zeros(n), np.zeros((n, 3))
+        self._check(lib().grains_gpu_get_masses(self.h, _d(m), _d(I), _d(wgt)))
+        return m, I, wgt
+
+    def record_pairs(self, on=True):
+        self._check(lib().grains_gpu_set_record_pairs(self.h, 1 if on else 0))
+
+    def pairs(self):
+        """Directed contact records (receiver id, partner id) of the last force evaluation."""
+        L = lib()
+        n = ctypes.c_size_t()
+        self._check(L.grains_gpu_get_pairs(self.h, None, 0, ctypes.byref(n)))
+        buf = np.zeros(2 * max(n.value, 1), dtype=np.int32)
+        self._check(L.grains_gpu_get_pairs(self.h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n.value,
+                                           ctypes.byref(n)))
+        return buf[: 2 * n.value].reshape(-1, 2).copy()
+
+    def grid(self):
+        dims = (ctypes.c_int * 3)()
+        edge = np.zeros(3)
+        self._check(lib().grains_gpu_grid(self.h, dims, _d(edge)))
+        return tuple(dims), edge
+
+    def recoveries(self):
+        n = ctypes.c_long()
+        self._check(lib().grains_gpu_recoveries(self.h, ctypes.byref(n)))
+        return n.value
