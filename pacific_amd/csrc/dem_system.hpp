@@ -13,6 +13,7 @@ struct grains_gpu_system {
   int device = 0;
   int nparticles = 0;
   double time = 0.;
+  double t_end = 0.;  // <TimeInterval End> of an XML-built system
   long steps_done = 0;
   bool initialized = false;
   int cur = 0;  // bit 0: position buffer, bit 1: velocity buffer

@@ -1,0 +1,599 @@
+// Grains3D XML input files -> a DEM system (grains_gpu_create_from_xml).
+//
+// A small self-contained XML reader (elements, attributes, text, comments,
+// processing instructions, DOCTYPE, CDATA, the five predefined entities)
+// replaces the Xerces DOM of GRAINS_V1/Grains/XML/src/ReaderXML.cpp; the
+// schema mapping follows the code that consumes the DOM in the reference:
+//   Grains::Construction ......... Base/src/Grains.cpp:535-1033
+//   Grains::Forces ............... Base/src/Grains.cpp:1039-1100
+//   Grains::AdditionalFeatures ... Base/src/Grains.cpp:1107-1960
+//   ContactBuilderFactory::define  Component/src/ContactBuilderFactory.cpp:82-172
+//   HookeContactForceModel::defineParameters  HookeContactForceModel.cpp:170-199
+//   Transform(DOMNode*) .......... Geometry/src/Transform.cpp:490-535
+// Supported: spheres, box obstacles (plain or inside <Composite>), Hooke
+// models, insertion at the initial time in file order from a position file or
+// a structured array, Zero / Constant / Random initial velocities.  Anything
+// else is rejected with GRAINS_CONFIG_ERROR and a message naming the node.
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/grains_gpu.h"
+#include "dem_system.hpp"
+
+namespace {
+
+struct XNode {
+  std::string name;
+  std::vector<std::pair<std::string, std::string>> attrs;
+  std::vector<XNode> kids;
+  std::string text;
+
+  const XNode* child(const char* n) const {
+    for (auto& k : kids)
+      if (k.name == n) return &k;
+    return nullptr;
+  }
+  std::vector<const XNode*> children(const char* n) const {
+    std::vector<const XNode*> out;
+    for (auto& k : kids)
+      if (k.name == n) out.push_back(&k);
+    return out;
+  }
+  bool has(const char* a) const {
+    for (auto& p : attrs)
+      if (p.first == a) return true;
+    return false;
+  }
+  std::string attr(const char* a) const {
+    for (auto& p : attrs)
+      if (p.first == a) return p.second;
+    return std::string();
+  }
+};
+
+struct XmlError {
+  std::string msg;
+};
+
+class Parser {
+ public:
+  explicit Parser(const std::string& s) : s_(s), i_(0) {}
+
+  XNode document() {
+    XNode root;
+    skip_misc();
+    if (i_ >= s_.size() || s_[i_] != '<') fail("no root element");
+    element(root);
+    skip_misc();
+    return root;
+  }
+
+ private:
+  const std::string& s_;
+  size_t i_;
+
+  [[noreturn]] void fail(const std::string& m) {
+    size_t line = 1;
+    for (size_t k = 0; k < i_ && k < s_.size(); ++k) line += s_[k] == '\n';
+    throw XmlError{"XML line " + std::to_string(line) + ": " + m};
+  }
+  bool starts(const char* p) const { return s_.compare(i_, std::strlen(p), p) == 0; }
+  void skip_ws() {
+    while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
+  }
+  void skip_until(const char* end) {
+    size_t k = s_.find(end, i_);
+    if (k == std::string::npos) fail(std::string("unterminated construct, expected ") + end);
+    i_ = k + std::strlen(end);
+  }
+  void skip_doctype() {
+    i_ += 9;  // "<!DOCTYPE"
+    int depth = 0;
+    while (i_ < s_.size()) {
+      char c = s_[i_++];
+      if (c == '[') ++depth;
+      else if (c == ']') --depth;
+      else if (c == '>' && depth <= 0) return;
+    }
+    fail("unterminated DOCTYPE");
+  }
+  // whitespace, comments, processing instructions, DOCTYPE
+  void skip_misc() {
+    for (;;) {
+      skip_ws();
+      if (starts("<?")) skip_until("?>");
+      else if (starts("<!--")) skip_until("-->");
+      else if (starts("<!DOCTYPE")) skip_doctype();
+      else return;
+    }
+  }
+  std::string name() {
+    size_t b = i_;
+    while (i_ < s_.size() && (std::isalnum((unsigned char)s_[i_]) || s_[i_] == '_' || s_[i_] == '-' ||
+                              s_[i_] == '.' || s_[i_] == ':'))
+      ++i_;
+    if (b == i_) fail("expected a name");
+    return s_.substr(b, i_ - b);
+  }
+  static std::string decode(const std::string& t) {
+    std::string o;
+    for (size_t k = 0; k < t.size(); ++k) {
+      if (t[k] != '&') { o += t[k]; continue; }
+      size_t e = t.find(';', k);
+      if (e == std::string::npos) { o += t[k]; continue; }
+      std::string ent = t.substr(k + 1, e - k - 1);
+      if (ent == "lt") o += '<';
+      else if (ent == "gt") o += '>';
+      else if (ent == "amp") o += '&';
+      else if (ent == "quot") o += '"';
+      else if (ent == "apos") o += '\'';
+      else if (!ent.empty() && ent[0] == '#') {
+        long c = ent.size() > 1 && (ent[1] == 'x' || ent[1] == 'X') ? std::strtol(ent.c_str() + 2, nullptr, 16)
+                                                                      : std::strtol(ent.c_str() + 1, nullptr, 10);
+        if (c > 0 && c < 128) o += (char)c;
+      } else {
+        o += t.substr(k, e - k + 1);
+      }
+      k = e;
+    }
+    return o;
+  }
+  void element(XNode& n) {
+    ++i_;  // '<'
+    n.name = name();
+    for (;;) {
+      skip_ws();
+      if (i_ >= s_.size()) fail("unterminated tag <" + n.name + ">");
+      if (starts("/>")) { i_ += 2; return; }
+      if (s_[i_] == '>') { ++i_; break; }
+      std::string a = name();
+      skip_ws();
+      if (i_ >= s_.size() || s_[i_] != '=') fail("expected '=' after attribute " + a);
+      ++i_;
+      skip_ws();
+      if (i_ >= s_.size() || (s_[i_] != '"' && s_[i_] != '\'')) fail("expected a quoted value for " + a);
+      char q = s_[i_++];
+      size_t e = s_.find(q, i_);
+      if (e == std::string::npos) fail("unterminated attribute value");
+      n.attrs.push_back({a, decode(s_.substr(i_, e - i_))});
+      i_ = e + 1;
+    }
+    // content
+    for (;;) {
+      if (i_ >= s_.size()) fail("element <" + n.name + "> not closed");
+      if (starts("</")) {
+        i_ += 2;
+        std::string nm = name();
+        if (nm != n.name) fail("closing </" + nm + "> does not match <" + n.name + ">");
+        skip_ws();
+        if (i_ >= s_.size() || s_[i_] != '>') fail("expected '>'");
+        ++i_;
+        return;
+      }
+      if (starts("<!--")) { skip_until("-->"); continue; }
+      if (starts("<?")) { skip_until("?>"); continue; }
+      if (starts("<![CDATA[")) {
+        i_ += 9;
+        size_t e = s_.find("]]>", i_);
+        if (e == std::string::npos) fail("unterminated CDATA");
+        n.text += s_.substr(i_, e - i_);
+        i_ = e + 3;
+        continue;
+      }
+      if (s_[i_] == '<') {
+        n.kids.emplace_back();
+        element(n.kids.back());
+        continue;
+      }
+      size_t e = s_.find('<', i_);
+      if (e == std::string::npos) e = s_.size();
+      n.text += decode(s_.substr(i_, e - i_));
+      i_ = e;
+    }
+  }
+};
+
+std::string trim(const std::string& s) {
+  size_t b = 0, e = s.size();
+  while (b < e && std::isspace((unsigned char)s[b])) ++b;
+  while (e > b && std::isspace((unsigned char)s[e - 1])) --e;
+  return s.substr(b, e - b);
+}
+
+struct Cfg {
+  std::string err;
+  bool ok = true;
+  void bad(const std::string& m) {
+    if (ok) err = m;
+    ok = false;
+  }
+};
+
+// ReaderXML::getNodeAttr_Double / getNodeValue_Double
+double dattr(const XNode* n, const char* a, Cfg& c) {
+  if (!n || !n->has(a)) {
+    c.bad(std::string("missing attribute ") + a + (n ? " in <" + n->name + ">" : std::string()));
+    return 0.;
+  }
+  std::string v = trim(n->attr(a));
+  char* end = nullptr;
+  double d = std::strtod(v.c_str(), &end);
+  if (end == v.c_str()) c.bad("attribute " + std::string(a) + " of <" + n->name + "> is not a number");
+  return d;
+}
+double dvalue(const XNode* n, Cfg& c, const char* what) {
+  if (!n) {
+    c.bad(std::string("missing node <") + what + ">");
+    return 0.;
+  }
+  std::string v = trim(n->text);
+  char* end = nullptr;
+  double d = std::strtod(v.c_str(), &end);
+  if (end == v.c_str()) c.bad("<" + n->name + "> does not hold a number");
+  return d;
+}
+
+// <AngularPosition Type="Identity|Matrix">  (Transform.cpp:510-530)
+bool basis_of(const XNode* owner, double B[9], Cfg& c) {
+  for (int i = 0; i < 9; ++i) B[i] = (i % 4) == 0 ? 1. : 0.;
+  const XNode* ap = owner->child("AngularPosition");
+  if (!ap) return true;
+  std::string mode = ap->attr("Type");
+  if (mode == "Matrix") {
+    std::istringstream in(ap->text);
+    for (int i = 0; i < 9; ++i)
+      if (!(in >> B[i])) {
+        c.bad("AngularPosition matrix needs 9 values");
+        return false;
+      }
+    // Matrix::isRotation: orthonormal with determinant +1
+    double det = B[0] * (B[4] * B[8] - B[5] * B[7]) - B[1] * (B[3] * B[8] - B[5] * B[6]) +
+                 B[2] * (B[3] * B[7] - B[4] * B[6]);
+    if (std::fabs(det - 1.) > 1e-8) {
+      c.bad("a matrix in one of the AngularPosition nodes is not a rotation matrix");
+      return false;
+    }
+  }
+  return true;
+}
+
+struct Material {
+  std::map<std::string, int> index;
+  int define(const std::string& name, Cfg& c) {  // ContactBuilderFactory::defineMaterial
+    auto it = index.find(name);
+    if (it != index.end()) return it->second;
+    int k = (int)index.size();
+    if (k >= dem::kMaxMaterials) c.bad("too many materials");
+    index[name] = k;
+    return k;
+  }
+};
+
+struct SphereClass {
+  int number;
+  double density, crust, radius;
+  int mat;
+};
+
+}  // namespace
+
+extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, grains_gpu_system** out) {
+  if (!xml_path || !out) return GRAINS_ARGUMENT_ERROR;
+  *out = nullptr;
+  std::ifstream f(xml_path);
+  if (!f) {
+    grains_gpu_config cfg0{};
+    grains_gpu_create(&cfg0, out);
+    (*out)->last_error = std::string("cannot open ") + xml_path;
+    return GRAINS_CONFIG_ERROR;
+  }
+  std::stringstream buf;
+  buf << f.rdbuf();
+  std::string src = buf.str();
+
+  grains_gpu_config cfg{};
+  cfg.cell_size_factor = 1.;
+  cfg.precision = precision;
+  cfg.scheme = GRAINS_LEAPFROG;
+  Cfg c;
+  XNode root;
+  try {
+    root = Parser(src).document();
+  } catch (XmlError& e) {
+    c.bad(e.msg);
+  }
+  std::string dir;
+  {
+    std::string p(xml_path);
+    size_t k = p.find_last_of('/');
+    dir = k == std::string::npos ? std::string() : p.substr(0, k + 1);
+  }
+
+  Material mats;
+  std::vector<SphereClass> classes;
+  struct BoxDef {
+    double L[3], centre[3], B[9], crust;
+    int mat;
+  };
+  std::vector<BoxDef> boxes;
+  struct ModelDef {
+    int a, b;
+    double kn, en, etat, muc, kr;
+  };
+  std::vector<ModelDef> models;
+  std::vector<double> positions;
+  double vconst[3] = {0, 0, 0}, wconst[3] = {0, 0, 0};
+  int vmode = 0;  // 0 zero, 1 constant, 2 random
+  double amp_t = 0., amp_r = 0.;
+  unsigned seed = 0;
+  double t_end = 0.;
+
+  if (c.ok && root.name != "Grains3D") c.bad("root element must be <Grains3D>");
+  if (c.ok && root.has("Type") && root.attr("Type") != "Standard")
+    c.bad("only <Grains3D Type=\"Standard\"> is supported");
+  const XNode* cons = c.ok ? root.child("Construction") : nullptr;
+  if (c.ok && !cons) c.bad("missing <Construction>");
+
+  // ---- Construction (Grains.cpp:535-1033)
+  if (c.ok) {
+    const XNode* lc = cons->child("LinkedCell");
+    cfg.max[0] = dattr(lc, "MX", c);
+    cfg.max[1] = dattr(lc, "MY", c);
+    cfg.max[2] = dattr(lc, "MZ", c);
+    if (lc && lc->has("CellSizeFactor")) cfg.cell_size_factor = dattr(lc, "CellSizeFactor", c);
+    if (const XNode* o = cons->child("Origin")) {
+      cfg.origin[0] = dattr(o, "OX", c);
+      cfg.origin[1] = dattr(o, "OY", c);
+      cfg.origin[2] = dattr(o, "OZ", c);
+    }
+    if (const XNode* p = cons->child("Periodicity")) {
+      const char* ax[3] = {"PX", "PY", "PZ"};
+      for (int a = 0; a < 3; ++a) cfg.periodic[a] = std::atoi(p->attr(ax[a]).c_str()) == 1 ? 1 : 0;
+    }
+    if (cons->child("PartialPeriodicity")) c.bad("<PartialPeriodicity> is not supported");
+    if (cons->child("Reload")) c.bad("<Reload> is not supported by the XML front-end; use grains_gpu_set_state");
+    if (cons->child("CompositeParticles")) c.bad("<CompositeParticles> are not supported");
+    if (const XNode* ps = cons->child("Particles")) {
+      for (const XNode* p : ps->children("Particle")) {
+        SphereClass sc;
+        sc.number = std::atoi(p->attr("Number").c_str());
+        sc.density = dattr(p, "Density", c);
+        const XNode* cv = p->child("Convex");
+        if (!cv) { c.bad("<Particle> without <Convex>"); break; }
+        sc.crust = dattr(cv, "CrustThickness", c);
+        const XNode* sph = cv->child("Sphere");
+        if (!sph) { c.bad("only <Sphere> particles are supported"); break; }
+        sc.radius = dattr(sph, "Radius", c);
+        const XNode* m = p->child("Material");
+        sc.mat = mats.define(m ? trim(m->text) : std::string(""), c);
+        classes.push_back(sc);
+      }
+    }
+    if (const XNode* os = cons->child("Obstacles")) {
+      // ObstacleBuilderFactory::create: <Composite> nests obstacles
+      std::vector<const XNode*> stack;
+      for (auto it = os->kids.rbegin(); it != os->kids.rend(); ++it) stack.push_back(&*it);
+      std::vector<const XNode*> flat;
+      while (!stack.empty()) {
+        const XNode* n = stack.back();
+        stack.pop_back();
+        if (n->name == "Composite") {
+          for (auto it = n->kids.rbegin(); it != n->kids.rend(); ++it)
+            if (it->name == "Composite" || it->name == "Obstacle") stack.push_back(&*it);
+        } else if (n->name == "Obstacle") {
+          flat.push_back(n);
+        } else {
+          c.bad("obstacle type <" + n->name + "> is not supported");
+        }
+      }
+      for (const XNode* n : flat) {
+        BoxDef b;
+        const XNode* cv = n->child("Convex");
+        if (!cv) { c.bad("<Obstacle> without <Convex>"); break; }
+        b.crust = dattr(cv, "CrustThickness", c);
+        const XNode* bx = cv->child("Box");
+        if (!bx) { c.bad("only <Box> obstacles are supported"); break; }
+        b.L[0] = dattr(bx, "LX", c);
+        b.L[1] = dattr(bx, "LY", c);
+        b.L[2] = dattr(bx, "LZ", c);
+        const XNode* ce = n->child("Centre");
+        b.centre[0] = dattr(ce, "X", c);
+        b.centre[1] = dattr(ce, "Y", c);
+        b.centre[2] = dattr(ce, "Z", c);
+        basis_of(n, b.B, c);
+        const XNode* m = n->child("Material");
+        b.mat = mats.define(m ? trim(m->text) : std::string(""), c);
+        boxes.push_back(b);
+      }
+    }
+    if (const XNode* cm = cons->child("ContactForceModels")) {
+      for (const XNode& k : cm->kids) {
+        ModelDef md;
+        const XNode* law = nullptr;
+        if (k.name == "Default") {
+          md.a = md.b = -1;
+        } else if (k.name == "ContactForceModel") {
+          const XNode* m = k.child("Material");
+          if (!m) { c.bad("<ContactForceModel> without <Material>"); break; }
+          md.a = mats.define(m->attr("materialA"), c);
+          md.b = mats.define(m->attr("materialB"), c);
+        } else {
+          continue;
+        }
+        for (const XNode& t : k.kids) {
+          if (t.name == "Material") continue;
+          if (t.name == "Hooke") law = &t;
+          else c.bad("contact force model <" + t.name + "> is not supported (only <Hooke>)");
+        }
+        if (!law) { c.bad("no contact force model in <" + k.name + ">"); break; }
+        md.kn = dvalue(law->child("kn"), c, "kn");
+        md.en = dvalue(law->child("en"), c, "en");
+        md.muc = dvalue(law->child("muc"), c, "muc");
+        md.kr = dvalue(law->child("kr"), c, "kr");
+        md.etat = law->child("etat") ? dvalue(law->child("etat"), c, "etat") : -1.;
+        models.push_back(md);
+      }
+    }
+  }
+
+  // ---- Forces (Grains.cpp:1039-1100): gravity is mandatory
+  if (c.ok) {
+    const XNode* fo = root.child("Forces");
+    const XNode* g = fo ? fo->child("Gravity") : nullptr;
+    if (!g) c.bad("missing <Forces><Gravity>");
+    else {
+      cfg.gravity[0] = dattr(g, "GX", c);
+      cfg.gravity[1] = dattr(g, "GY", c);
+      cfg.gravity[2] = dattr(g, "GZ", c);
+    }
+  }
+
+  // ---- Simulation (Grains.cpp:1107-1960)
+  if (c.ok) {
+    const XNode* sim = root.child("Simulation");
+    if (!sim) c.bad("missing <Simulation>");
+    else {
+      const XNode* ti = sim->child("TimeInterval");
+      cfg.t_start = dattr(ti, "Start", c);
+      t_end = dattr(ti, "End", c);
+      cfg.dt = dattr(sim->child("TimeStep"), "dt", c);
+      if (const XNode* tint = sim->child("TimeIntegration")) {
+        std::string t = tint->attr("Type");
+        if (t == "SecondOrderLeapFrog") cfg.scheme = GRAINS_LEAPFROG;
+        else if (t == "FirstOrderExplicit") cfg.scheme = GRAINS_FIRST_ORDER_EXPLICIT;
+        else if (t == "SecondOrderExplicit") cfg.scheme = GRAINS_SECOND_ORDER_EXPLICIT;
+        else if (t == "SecondOrderAdamsBashforth") cfg.scheme = GRAINS_ADAMS_BASHFORTH2;
+        else c.bad("unknown time integrator " + t);
+      }
+      const XNode* ins = sim->child("ParticleInsertion");
+      size_t nnew = 0;
+      for (auto& sc : classes) nnew += (size_t)sc.number;
+      if (nnew && !ins) c.bad("particles declared without <ParticleInsertion>");
+      if (ins && c.ok) {
+        const XNode* mode = ins->child("Mode");
+        if (!mode || mode->attr("Type") != "InitialTime")
+          c.bad("only <Mode Type=\"InitialTime\"> insertion is supported");
+        const XNode* order = ins->child("Order");
+        if (order && order->attr("Type") == "Random") c.bad("only ordered insertion is supported");
+        const XNode* iap = ins->child("InitialAngularPosition");
+        if (iap && (iap->attr("Type") == "Random" || iap->attr("Type") == "File"))
+          c.bad("only fixed initial angular positions are supported");
+        if (const XNode* rgs = ins->child("RandomGeneratorSeed")) {
+          std::string t = rgs->attr("Type");
+          if (t == "UserDefined") seed = (unsigned)std::atoi(rgs->attr("Value").c_str());
+          else if (t == "Random") c.bad("RandomGeneratorSeed Type=\"Random\" is not reproducible");
+        }
+        const XNode* pp = ins->child("ParticlePosition");
+        if (!pp) c.bad("missing <ParticlePosition>");
+        else if (const XNode* fl = pp->child("File")) {
+          std::string name = fl->attr("Name");
+          std::ifstream pf(name);
+          if (!pf) pf.open(dir + name);
+          if (!pf) c.bad("cannot open position file " + name);
+          double x, y, z;
+          while (pf >> x >> y >> z) {  // Point3 operator>> (Grains.cpp:2243)
+            positions.push_back(x);
+            positions.push_back(y);
+            positions.push_back(z);
+          }
+        } else if (const XNode* st = pp->child("StructuredArray")) {
+          // Grains.cpp:1421-1463 and setPositionParticlesStructuredArray :2303-2345
+          const XNode* bx = st->child("Box");
+          const XNode* nb = st->child("Number");
+          if (!bx || bx->kids.size() < 2 || !nb) c.bad("<StructuredArray> needs <Box> with two points and <Number>");
+          else {
+            double A[3] = {dattr(&bx->kids[0], "X", c), dattr(&bx->kids[0], "Y", c), dattr(&bx->kids[0], "Z", c)};
+            double B[3] = {dattr(&bx->kids[1], "X", c), dattr(&bx->kids[1], "Y", c), dattr(&bx->kids[1], "Z", c)};
+            long NX = std::atol(nb->attr("NX").c_str()), NY = std::atol(nb->attr("NY").c_str()),
+                 NZ = std::atol(nb->attr("NZ").c_str());
+            if ((size_t)(NX * NY * NZ) != nnew) c.bad("structured array size differs from the number of particles");
+            double dx = (B[0] - A[0]) / double(NX), dy = (B[1] - A[1]) / double(NY), dz = (B[2] - A[2]) / double(NZ);
+            for (long k = 0; k < NX; ++k)
+              for (long l = 0; l < NY; ++l)
+                for (long m = 0; m < NZ; ++m) {
+                  positions.push_back(A[0] + (double(k) + 0.5) * dx);
+                  positions.push_back(A[1] + (double(l) + 0.5) * dy);
+                  positions.push_back(A[2] + (double(m) + 0.5) * dz);
+                }
+          }
+        } else {
+          c.bad("<ParticlePosition> needs <File> or <StructuredArray>");
+        }
+        if (c.ok && positions.size() / 3 < nnew) c.bad("fewer positions than particles to insert");
+        if (const XNode* iv = ins->child("InitialVelocity")) {
+          std::string m = iv->attr("Mode");
+          if (m == "Constant") {
+            vmode = 1;
+            if (const XNode* t = iv->child("TranslationalVelocity")) {
+              vconst[0] = dattr(t, "VX", c);
+              vconst[1] = dattr(t, "VY", c);
+              vconst[2] = dattr(t, "VZ", c);
+            }
+            if (const XNode* r = iv->child("AngularVelocity")) {
+              wconst[0] = dattr(r, "RX", c);
+              wconst[1] = dattr(r, "RY", c);
+              wconst[2] = dattr(r, "RZ", c);
+            }
+          } else if (m == "Random") {
+            vmode = 2;
+            if (const XNode* t = iv->child("Translational")) amp_t = dattr(t, "Amplitude", c);
+            if (const XNode* r = iv->child("Angular")) amp_r = dattr(r, "Amplitude", c);
+          }
+        }
+      }
+    }
+  }
+
+  grains_gpu_system* s = nullptr;
+  grains_gpu_create(&cfg, &s);
+  *out = s;
+  if (!c.ok) {
+    s->last_error = c.err;
+    return GRAINS_CONFIG_ERROR;
+  }
+  s->t_end = t_end;
+  int e;
+  for (auto& m : models) {
+    e = grains_gpu_set_contact_model(s, m.a, m.b, m.kn, m.en, m.etat, m.muc, m.kr);
+    if (e) return e;
+  }
+  // insertion at the initial time, class by class in file order
+  // (AllComponents::getParticleToInsert PM_ORDERED, Grains.cpp:2070-2140)
+  srand(seed);
+  size_t k = 0;
+  for (auto& sc : classes) {
+    for (int i = 0; i < sc.number; ++i, ++k) {
+      double v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+      if (vmode == 1) {
+        for (int a = 0; a < 3; ++a) { v[a] = vconst[a]; w[a] = wconst[a]; }
+      } else if (vmode == 2) {  // Grains::computeInitVit IV_RANDOM (Grains.cpp:2170-2200)
+        if (amp_t != 0.)
+          for (int a = 0; a < 3; ++a) v[a] = amp_t * (2. * (double(random()) / double(INT_MAX)) - 1.);
+        if (amp_r != 0.) {
+          w[2] = amp_r * (2. * (double(random()) / double(INT_MAX)) - 1.);
+          w[0] = amp_r * (2. * (double(random()) / double(INT_MAX)) - 1.);
+          w[1] = amp_r * (2. * (double(random()) / double(INT_MAX)) - 1.);
+        }
+      }
+      int mat = sc.mat;
+      e = grains_gpu_add_spheres(s, 1, &sc.radius, &sc.density, &sc.crust, &mat, &positions[3 * k], v, w);
+      if (e) return e;
+    }
+  }
+  for (auto& b : boxes) {
+    e = grains_gpu_add_box(s, b.L, b.centre, b.B, b.crust, b.mat);
+    if (e) return e;
+  }
+  return grains_gpu_initialize(s);
+}

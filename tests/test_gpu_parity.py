@@ -21,8 +21,13 @@ pytestmark = pytest.mark.gpu
 
 
 def settled_state(wl, steps):
+    """Oracle state after `steps` steps, with its forces re-evaluated at the
+    end-of-step velocities (inside a step the forces see the half-step
+    velocity), so that both sides evaluate the same (x, v, w)."""
     o = make_oracle(wl)
     o.step(steps)
+    so = o.state()
+    o.set_state(x=so["x"], v=so["v"], w=so["w"], q=so["q"], recompute=True)
     return o
 
 
@@ -39,9 +44,9 @@ def compare_forces(g, o, tol):
 
 
 def test_forces_and_pairs_on_settling_box(gpu):
-    # 1,000 spheres in the walled box after 4,000 oracle steps (bed + falling)
+    # 1,000 spheres in the walled box after 15,000 oracle steps (bed + falling)
     wl = workloads.settling_box(1000)
-    o = settled_state(wl, 4000)
+    o = settled_state(wl, 15000)
     so = o.state()
     g = make_gpu(wl)
     g.set_state(x=so["x"], v=so["v"], w=so["w"], q=so["q"], recompute=True)
@@ -53,8 +58,7 @@ def test_forces_and_pairs_on_settling_box(gpu):
 
 def test_forces_and_pairs_periodic_gas(gpu):
     wl = workloads.periodic_gas(8000, seed=4, sigma=0.5)
-    o = make_oracle(wl)
-    o.step(300)
+    o = settled_state(wl, 300)
     so = o.state()
     g = make_gpu(wl)
     g.set_state(x=so["x"], v=so["v"], w=so["w"], q=so["q"], recompute=True)
@@ -120,7 +124,7 @@ def test_other_integrators(gpu, scheme):
 
 def test_mixed_precision_force_error(gpu):
     wl = workloads.settling_box(1000)
-    o = settled_state(wl, 4000)
+    o = settled_state(wl, 15000)
     so = o.state()
     g = make_gpu(wl, precision=1)
     g.set_state(x=so["x"], v=so["v"], w=so["w"], q=so["q"], recompute=True)

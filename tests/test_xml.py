@@ -1,0 +1,148 @@
+"""Grains3D XML front-end (grains_gpu_create_from_xml): schema coverage,
+error reporting, and equivalence with the programmatic construction."""
+import os
+
+import numpy as np
+import pytest
+
+from pacific_amd import GrainsError, GrainsGPU, workloads
+from tests.conftest import gpu_available
+from tests.helpers import make_gpu
+
+XML_TEMPLATE = """<?xml version="1.0" encoding="iso-8859-1"?>
+<!DOCTYPE Grains3D SYSTEM "Grains3D.dtd">
+<!-- settling box, written by the test -->
+<Grains3D Type="Standard">
+  <Construction>
+    <LinkedCell MX="0.02" MY="0.02" MZ="0.06" CellSizeFactor="1."/>
+    <Origin OX="0." OY="0." OZ="0."/>
+    <Periodicity PX="0" PY="0" PZ="0"/>
+    <CollisionDetectionAlgorithm>
+      <CollisionDetection Type="GJK" Tolerance="1.e-10"/>
+    </CollisionDetectionAlgorithm>
+    <ContactForceModels>
+      <ContactForceModel>
+        <Material materialA="matP" materialB="matP"/>
+        <Hooke>
+          <kn>1.e4</kn>
+          <en>0.9</en>
+          <muc>0.5</muc>
+          <kr>0.</kr>
+        </Hooke>
+      </ContactForceModel>
+      <ContactForceModel>
+        <Material materialA="matP" materialB="matW"/>
+        <Hooke><kn>1.e4</kn><en>0.9</en><muc>0.5</muc><kr>0.</kr><etat>-1</etat></Hooke>
+      </ContactForceModel>
+    </ContactForceModels>
+    <Particles>
+      <Particle Number="{n}" Density="2500.">
+        <Convex CrustThickness="5.e-5">
+          <Sphere Radius="0.001"/>
+        </Convex>
+        <AngularPosition Type="Identity"></AngularPosition>
+        <Material>matP</Material>
+      </Particle>
+    </Particles>
+    <Obstacles>
+      <Composite name="Box">
+{walls}
+      </Composite>
+    </Obstacles>
+  </Construction>
+  <Forces>
+    <Gravity GX="0." GY="0." GZ="-9.81"/>
+  </Forces>
+  <Simulation>
+    <TimeInterval Start="0." End="0.1"/>
+    <TimeStep dt="1.e-5"/>
+    <TimeIntegration Type="SecondOrderLeapFrog"/>
+    <ParticleInsertion>
+      <Mode Type="InitialTime"/>
+      <Order Type="Ordered"/>
+      <InitialAngularPosition Type="Fixed as defined in input file particle class"/>
+      <RandomGeneratorSeed Type="Default"/>
+      <ParticlePosition>
+        <File Name="{posfile}"/>
+      </ParticlePosition>
+      <InitialVelocity Mode="Zero"/>
+    </ParticleInsertion>
+  </Simulation>
+</Grains3D>
+"""
+
+WALL = """        <Obstacle name="w{i}">
+          <Convex CrustThickness="5.e-5">
+            <Box LX="{L[0]!r}" LY="{L[1]!r}" LZ="{L[2]!r}"/>
+          </Convex>
+          <Centre X="{c[0]!r}" Y="{c[1]!r}" Z="{c[2]!r}"/>
+          <AngularPosition Type="Identity"></AngularPosition>
+          <Material>matW</Material>
+        </Obstacle>"""
+
+
+def write_case(tmp_path, n=200, **subs):
+    wl = workloads.settling_box(n)
+    pos = tmp_path / "positions.result"
+    np.savetxt(pos, wl["x"], fmt="%.17g")
+    walls = "\n".join(WALL.format(i=i, L=[float(v) for v in b[0]], c=[float(v) for v in b[1]])
+                      for i, b in enumerate(wl["boxes"]))
+    text = XML_TEMPLATE.format(n=n, walls=walls, posfile="positions.result")
+    for k, v in subs.items():
+        text = text.replace(k, v)
+    p = tmp_path / "insert.xml"
+    p.write_text(text)
+    return str(p), wl
+
+
+def expect_error(path, kind, fragment):
+    with pytest.raises(GrainsError) as e:
+        GrainsGPU.from_xml(path)
+    assert e.value.kind == kind, str(e.value)
+    assert fragment in str(e.value)
+
+
+def test_valid_file_parses(tmp_path):
+    path, _ = write_case(tmp_path)
+    if gpu_available():
+        g = GrainsGPU.from_xml(path)
+        assert g.n == 200
+    else:
+        # parsing succeeded; initialisation needs the device
+        expect_error(path, "DeviceError", "")
+
+
+@pytest.mark.parametrize("sub,frag", [
+    (("<Hooke>", "<HookeMemory>", "</Hooke>", "</HookeMemory>"), "not supported"),
+    (("<Sphere Radius=\"0.001\"/>", "<Cylinder Radius=\"0.001\" Height=\"0.002\"/>"), "only <Sphere>"),
+    (("<Mode Type=\"InitialTime\"/>", "<Mode Type=\"OverTime\"/>"), "InitialTime"),
+    (("<Gravity GX=\"0.\" GY=\"0.\" GZ=\"-9.81\"/>", ""), "Gravity"),
+    (("<TimeStep dt=\"1.e-5\"/>", "<TimeStep/>"), "missing attribute dt"),
+    (("</Construction>", "</Construktion>"), "does not match"),
+    (("positions.result", "missing.result"), "cannot open position file"),
+])
+def test_rejections(tmp_path, sub, frag):
+    path, _ = write_case(tmp_path, **dict(zip(sub[::2], sub[1::2])))
+    expect_error(path, "ConfigError", frag)
+
+
+def test_missing_contact_model(tmp_path):
+    path, _ = write_case(tmp_path, **{'materialB="matW"': 'materialB="matX"'})
+    # matP-matW has no model any more
+    kind = "ConfigError"
+    with pytest.raises(GrainsError) as e:
+        GrainsGPU.from_xml(path)
+    assert e.value.kind == kind and "no contact force model" in str(e.value)
+
+
+@pytest.mark.gpu
+def test_xml_equals_programmatic_build(tmp_path, gpu):
+    path, wl = write_case(tmp_path, n=300)
+    wl = workloads.settling_box(300)
+    g1 = GrainsGPU.from_xml(path)
+    g2 = make_gpu(wl, record=False)
+    g1.step(200)
+    g2.step(200)
+    s1, s2 = g1.state(), g2.state()
+    for k in ("x", "v", "w", "q"):
+        assert np.array_equal(s1[k], s2[k]), k
