@@ -94,7 +94,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=400)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        # a plain copy of known size: the rocprofv3 FETCH_SIZE / WRITE_SIZE of
+        # this kernel calibrates the counters used for the force sweep
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "reps": 5}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the DEM step on one GPU only (no sharding, no collectives); "
