@@ -70,6 +70,32 @@ def cpu_baseline(wl, steps):
     return wl["radius"].size * steps / (t1 - t0), t1 - t0
 
 
+def batched_leg(args, prec):
+    """512 independent settling boxes (sizes log-uniform in [200, 5000],
+    per-system mu and en) in one persistent launch per hipGraph replay, the
+    workgroup size picked by the autotuner."""
+    from pacific_amd import GrainsBatch, GrainsGPU, workloads
+    ens = workloads.ensemble(512, seed=5)
+    members = []
+    for wl in ens:
+        g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], 0.0, wl["dt"],
+                      0, prec)
+        workloads.build(g, wl)
+        members.append(g)
+    b = GrainsBatch(members, precision=prec)
+    del members
+    best, tune = b.autotune(20)
+    chunk = 100
+    b.step(chunk, chunk=chunk)
+    ms = b.step_timed(args.batch_steps, chunk=chunk)
+    ntot = b.n_total
+    value = ntot * args.batch_steps / (ms / 1e3)
+    log(f"bench: batched ensemble 512 systems, {ntot} particles, workgroup {best}: {value:.4g} particle-steps/s")
+    return {"batched_value": value, "batched_config": {"systems": 512, "particles": ntot, "steps": args.batch_steps,
+                                                        "workgroup": best, "graph_chunk": chunk,
+                                                        "autotune_ms": {str(k): v for k, v in tune.items()}}}
+
+
 def read_traffic(name):
     """HBM bytes per launch of the force sweep from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
@@ -94,6 +120,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=400)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -148,6 +176,9 @@ def main():
         extra["fp64_ms_per_step"] = ms64 / max(args.steps // 2, 1)
         del g64
         log(f"bench: FP64 path {extra['fp64_value']:.4g} particle-steps/s")
+
+    if not args.no_batch:
+        extra.update(batched_leg(args, prec))
 
     traffic = read_traffic(f"traffic_forces_{args.precision}_{args.n}.json")
     out = {

@@ -155,6 +155,37 @@ int grains_gpu_error_ids(grains_gpu_system* sys, int* id0, int* id1);
 int grains_gpu_recoveries(grains_gpu_system* sys, long* n);
 const char* grains_gpu_last_error(const grains_gpu_system* sys);
 
+/* ---- Batched ensembles ---------------------------------------------------
+ * Many independent systems (each with its own box, obstacles, contact
+ * parameters and size) advanced by one persistent launch, one workgroup per
+ * system.  No counterpart in the reference, which runs one system per process
+ * (static state in GrainsExec, ContactBuilderFactory and Cell); each member
+ * evolves exactly as grains_gpu_step would evolve it alone. */
+typedef struct grains_gpu_batch grains_gpu_batch;
+
+/* Members are configured with grains_gpu_create / set_contact_model /
+ * add_spheres / add_box and NOT initialised; the batch copies their data (the
+ * members may be destroyed afterwards).  SecondOrderLeapFrog only.  Bins every
+ * member and computes its initial forces. */
+int grains_gpu_batch_create(grains_gpu_system* const* systems, int nsys, int precision, grains_gpu_batch** out);
+void grains_gpu_batch_destroy(grains_gpu_batch* b);
+/* workgroup size per system: 64, 128, 256 or 512 */
+int grains_gpu_batch_set_block(grains_gpu_batch* b, int block);
+/* nsteps steps of every member; whole chunks of `chunk` steps replay one
+ * captured hipGraph of the persistent launch (chunk <= 0: direct launches) */
+int grains_gpu_batch_step(grains_gpu_batch* b, int nsteps, int chunk);
+int grains_gpu_batch_step_timed(grains_gpu_batch* b, int nsteps, int chunk, double* ms);
+/* times trial_steps steps at each workgroup size (64, 128, 256, 512 ->
+ * ms_out[0..3]), keeps the fastest; advances the members by 4 * trial_steps */
+int grains_gpu_batch_autotune(grains_gpu_batch* b, int trial_steps, int* best_block, double* ms_out);
+/* particles of member isys, or of the whole batch for isys < 0 */
+size_t grains_gpu_batch_num_particles(const grains_gpu_batch* b, int isys);
+double grains_gpu_batch_time(const grains_gpu_batch* b);
+/* state of member isys in its own id order (NULL pointers skipped) */
+int grains_gpu_batch_get_state(grains_gpu_batch* b, int isys, double* x, double* v, double* w, double* q, double* F,
+                               double* T);
+const char* grains_gpu_batch_last_error(const grains_gpu_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,481 +20,10 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
-#include "dem_math.hpp"
-#include "dem_types.hpp"
+#include "dem_device.hpp"
 #include "dem_launch.hpp"
 
 namespace dem {
-
-// ---------------------------------------------------------------------------
-// FP32 helpers for the mixed force path
-struct f3 {
-  float x, y, z;
-};
-DEM_HD f3 f3of(d3 a) { return f3{(float)a.x, (float)a.y, (float)a.z}; }
-DEM_HD f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-DEM_HD f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-DEM_HD f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
-DEM_HD f3 operator*(f3 a, float d) { return f3{a.x * d, a.y * d, a.z * d}; }
-DEM_HD f3 operator*(float d, f3 a) { return f3{a.x * d, a.y * d, a.z * d}; }
-DEM_HD f3& operator+=(f3& a, f3 b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
-DEM_HD float dotf(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
-DEM_HD f3 crossf(f3 a, f3 b) {
-  return f3{fmaf(a.y, b.z, -a.z * b.y), fmaf(a.z, b.x, -a.x * b.z), fmaf(a.x, b.y, -a.y * b.x)};
-}
-DEM_HD float normf(f3 a) { return sqrtf(dotf(a, a)); }
-
-// ---------------------------------------------------------------------------
-// Workgroup -> tile mapping.  Consecutive workgroups are dispatched
-// round-robin over the 8 XCDs; remapping gives each XCD one contiguous range
-// of sorted particles, so neighbouring cells are fetched into the same L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  int xcd = b & 7, idx = b >> 3;
-  int per = nb >> 3, rem = nb & 7;
-  return xcd * per + (xcd < rem ? xcd : rem) + idx;
-}
-
-__device__ __forceinline__ void raise_error(DevStatus* st, int code, int id0, int id1) {
-  if (atomicCAS(&st->code, 0, code) == 0) {
-    st->id0 = id0;
-    st->id1 = id1;
-  }
-}
-
-// Cell::GetCell (Cell.cpp:172-184) on one axis, in reference indexing
-__device__ __forceinline__ int ref_cell(double x, const Grid& g, int a) {
-  return int(floor((x - g.lo[a]) / g.edge[a]));
-}
-
-// ---------------------------------------------------------------------------
-// Narrow phase (FP64)
-
-// ClosestPointSPHERESPHERE (RigidBodyWithCrust.cpp:596-640) with A = self.
-// The caller has already computed AB = xB - xA and nab = Norm(AB).
-// Returns 1 contact, 0 none, -1 ContactError (overlap >= crust sum).
-DEM_HD int sphere_sphere(d3 xA, double rA, double cA, d3 AB, double nab, double rB, double cB, d3& P, d3& ov,
-                         double& d) {
-  double distance = nab - (rA + rB);
-  if (distance > 0.) return 0;
-  if (-distance >= cA + cB) return -1;
-  P = xA + ((rA + 0.5 * distance) * AB) / nab;
-  ov = distance * (AB / nab);
-  d = distance;
-  return 1;
-}
-
-DEM_HD d3 corner_of(const BoxObstacle& o, int n) {
-  return d3{o.corners[3 * n], o.corners[3 * n + 1], o.corners[3 * n + 2]};
-}
-DEM_HD double comp(d3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-DEM_HD void set_comp(d3& a, int i, double v) {
-  if (i == 0) a.x = v; else if (i == 1) a.y = v; else a.z = v;
-}
-
-// Box::ContactCornerSPHERE (Box.cpp:725-744)
-DEM_HD void box_corner(const BoxObstacle& o, d3 C, double R, int n, double& overlap, d3& cp) {
-  d3 dist = C - corner_of(o, n);
-  double nd = norm(dist);
-  if (nd < R) {
-    overlap = nd - R;
-    cp = C - (1. + 0.5 * overlap / nd) * dist;
-  }
-}
-// Box::ContactEdgeSPHERE (Box.cpp:750-769)
-DEM_HD void box_edge(const BoxObstacle& o, d3 C, double R, int n, int dir, double& overlap, d3& cp) {
-  d3 proj = C;
-  d3 cn = corner_of(o, n);
-  set_comp(proj, dir, comp(cn, dir));
-  d3 dist = proj - cn;
-  double nd = norm(dist);
-  if (nd < R) {
-    overlap = nd - R;
-    cp = C - (1. + 0.5 * overlap / nd) * dist;
-  }
-}
-// Box::IntersectionPointSPHERE (Box.cpp:466-716); false if the centre lies in
-// the box (a ContactError in the reference)
-DEM_HD bool box_sphere(const BoxObstacle& o, d3 C, double R, double& overlap, d3& cp) {
-  const double ex = o.extent[0], ey = o.extent[1], ez = o.extent[2];
-  const double gx = C.x, gy = C.y, gz = C.z;
-  overlap = 1.;
-  cp = d3{0., 0., 0.};
-  if (gx > ex) {
-    if (gy > ey) {
-      if (gz > ez) box_corner(o, C, R, 6, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 5, overlap, cp);
-      else box_edge(o, C, R, 5, 2, overlap, cp);
-    } else if (gy < -ey) {
-      if (gz > ez) box_corner(o, C, R, 2, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 1, overlap, cp);
-      else box_edge(o, C, R, 1, 2, overlap, cp);
-    } else {
-      if (gz > ez) box_edge(o, C, R, 6, 1, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 1, 1, overlap, cp);
-      else {
-        double nd = gx - ex;
-        if (nd < R) { overlap = nd - R; cp = d3{ex + 0.5 * overlap, C.y, C.z}; }
-      }
-    }
-  } else if (gx < -ex) {
-    if (gy > ey) {
-      if (gz > ez) box_corner(o, C, R, 7, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 4, overlap, cp);
-      else box_edge(o, C, R, 4, 2, overlap, cp);
-    } else if (gy < -ey) {
-      if (gz > ez) box_corner(o, C, R, 3, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 0, overlap, cp);
-      else box_edge(o, C, R, 0, 2, overlap, cp);
-    } else {
-      if (gz > ez) box_edge(o, C, R, 3, 1, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 0, 1, overlap, cp);
-      else {
-        double nd = -ex - gx;
-        if (nd < R) { overlap = nd - R; cp = d3{-ex - 0.5 * overlap, C.y, C.z}; }
-      }
-    }
-  } else {
-    if (gy > ey) {
-      if (gz > ez) box_edge(o, C, R, 6, 0, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 4, 0, overlap, cp);
-      else {
-        double nd = gy - ey;
-        if (nd < R) { overlap = nd - R; cp = d3{C.x, ey + 0.5 * overlap, C.z}; }
-      }
-    } else if (gy < -ey) {
-      if (gz > ez) box_edge(o, C, R, 3, 0, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 0, 0, overlap, cp);
-      else {
-        double nd = -ey - gy;
-        if (nd < R) { overlap = nd - R; cp = d3{C.x, -ey - 0.5 * overlap, C.z}; }
-      }
-    } else {
-      if (gz > ez) {
-        double nd = gz - ez;
-        if (nd < R) { overlap = nd - R; cp = d3{C.x, C.y, ez + 0.5 * overlap}; }
-      } else if (gz < -ez) {
-        double nd = -ez - gz;
-        if (nd < R) { overlap = nd - R; cp = d3{C.x, C.y, -ez - 0.5 * overlap}; }
-      } else {
-        return false;
-      }
-    }
-  }
-  return true;
-}
-
-// Transform::operator() (Transform.cpp:310-316)
-DEM_HD d3 apply_tf(const double* B, const double* o, d3 p) {
-  return d3{dot(d3{B[0], B[1], B[2]}, p) + o[0], dot(d3{B[3], B[4], B[5]}, p) + o[1],
-            dot(d3{B[6], B[7], B[8]}, p) + o[2]};
-}
-
-// ClosestPointSPHEREBOX with the sphere as body A (RigidBodyWithCrust.cpp:647-693)
-DEM_HD int sphere_box(const BoxObstacle& o, d3 xA, double rA, double cA, d3& P, d3& ov, double& d) {
-  double overlap = 0.;
-  d3 local = apply_tf(o.Binv, o.invorigin, xA);
-  d3 cpl;
-  if (!box_sphere(o, local, rA, overlap, cpl)) return -1;
-  if (!(overlap < 0.)) return 0;
-  if (-overlap >= cA + o.crust) return -1;
-  P = apply_tf(o.B, o.centre, cpl);
-  d3 AB = P - xA;
-  ov = (overlap / norm(AB)) * AB;
-  d = overlap;
-  return 1;
-}
-
-// ContactError recovery, RigidBodyWithCrust::ClosestPoint_ErreurHandling
-// (RigidBodyWithCrust.cpp:534-590): distance between the shapes shrunk by ten
-// crust thicknesses (scale capped at 0.5, :770-798), closed form for spheres
-// and boxes.  Returns false when the shrunk shapes still overlap.
-DEM_HD double shrink_scale(double extent, double crust) {
-  double sc = extent < kEps ? 1. : (extent - 10. * crust) / extent;
-  return sc > 0.5 ? sc : 0.5;
-}
-DEM_HD bool recover_ss(d3 xA, double rA, double cA, d3 xB, double rB, double cB, d3& P, d3& ov, double& d) {
-  double ra = rA * shrink_scale(rA, cA);
-  double rb = rB * shrink_scale(rB, cB);
-  d3 AB = xB - xA;
-  double nab = norm(AB);
-  double distance = nab - ra - rb;
-  if (distance < kEps) return false;
-  d3 n = AB / nab;
-  d3 pA = xA + ra * n, pB = xB - rb * n;
-  d3 ba = pA - pB;
-  ba /= norm(ba);
-  P = pA / 2.0 + pB / 2.0;
-  double imposed = 1. * (cA + cB);
-  ov = imposed * ba;
-  d = -imposed;
-  return true;
-}
-DEM_HD bool recover_sb(const BoxObstacle& o, d3 xA, double rA, double cA, d3& P, d3& ov, double& d) {
-  double ra = rA * shrink_scale(rA, cA);
-  double es[3];
-  for (int a = 0; a < 3; ++a) es[a] = o.extent[a] * shrink_scale(o.extent[a], o.crust);
-  d3 local = apply_tf(o.Binv, o.invorigin, xA);
-  d3 q{fmin(fmax(local.x, -es[0]), es[0]), fmin(fmax(local.y, -es[1]), es[1]), fmin(fmax(local.z, -es[2]), es[2])};
-  d3 dl = local - q;
-  double nd = norm(dl);
-  double distance = nd - ra;
-  if (distance < kEps) return false;
-  d3 pAl = local - (ra / nd) * dl;
-  d3 pA = apply_tf(o.B, o.centre, pAl), pB = apply_tf(o.B, o.centre, q);
-  d3 ba = pA - pB;
-  ba /= norm(ba);
-  P = pA / 2.0 + pB / 2.0;
-  double imposed = 1. * (cA + o.crust);
-  ov = imposed * ba;
-  d = -imposed;
-  return true;
-}
-
-// ---------------------------------------------------------------------------
-// Hooke force model, FP64 (HookeContactForceModel.cpp:44-114), p0 = self.
-// Returns the force on p0 and the rolling moment on p0.
-DEM_HD void hooke64(const HookeModel& m, d3 x0, d3 v0, d3 w0, double m0, double req0, d3 x1, d3 v1, d3 w1,
-                    double m1, double req1, d3 P, d3 pen, d3& f, d3& dM) {
-  d3 normal = pen;
-  normal /= norm(normal);
-  round_small(normal);
-  d3 tmpV = (v0 + cross(w0, P - x0)) - (v1 + cross(w1, P - x1));
-  d3 v_n = normal * dot(tmpV, normal);
-  d3 v_t = tmpV - v_n;
-  d3 delFN = m.kn * pen;
-  double avmass = 1. / (1. / m0 + 1. / m1);
-  double gamman = -2. * m.beta * sqrt(avmass * m.kn);
-  delFN -= gamman * v_n;
-  double normFN = norm(delFN);
-  double etat = m.etat == -1. ? -m.beta * sqrt(m.kn / avmass) : m.etat;
-  double gammat = 2. * etat * avmass;
-  d3 delFT = -gammat * v_t;
-  double fn = m.muc * normFN;
-  double ft = norm(delFT);
-  if (fn < ft) {
-    double nvt = norm(v_t);
-    d3 tangent{0., 0., 0.};
-    if (nvt > kEps) tangent = -v_t / nvt;
-    delFT = tangent * fn;
-  }
-  dM = d3{0., 0., 0.};
-  if (m.kr != 0.) {
-    d3 wrel = w0 - w1;
-    double nwrel = norm(wrel);
-    double nwtrel = norm(cross(w0, x0 - P) - cross(w1, x1 - P));
-    if (nwrel > kEps) {
-      double Req = 1. / (1. / req0 + 1. / req1);
-      dM = -(m.kr * Req * normFN * nwtrel / nwrel) * wrel;
-    }
-  }
-  double coef = 1.;
-  f = coef * (delFN + delFT);
-}
-
-// Same model in FP32 on lever arms relative to each body's centre.
-DEM_HD void hooke32(const HookeModel& m, f3 l0, f3 v0, f3 w0, float m0, float req0, f3 l1, f3 v1, f3 w1,
-                    float m1, float req1, f3 pen, f3& f, f3& dM) {
-  const float kn = (float)m.kn, beta = (float)m.beta;
-  float npen = normf(pen);
-  f3 normal = pen * (1.f / npen);
-  normal.x = fabsf(normal.x) < 1e-10f ? 0.f : normal.x;
-  normal.y = fabsf(normal.y) < 1e-10f ? 0.f : normal.y;
-  normal.z = fabsf(normal.z) < 1e-10f ? 0.f : normal.z;
-  f3 tmpV = (v0 + crossf(w0, l0)) - (v1 + crossf(w1, l1));
-  f3 v_n = normal * dotf(tmpV, normal);
-  f3 v_t = tmpV - v_n;
-  float avmass = 1.f / (1.f / m0 + 1.f / m1);
-  float gamman = -2.f * beta * sqrtf(avmass * kn);
-  f3 delFN = kn * pen - gamman * v_n;
-  float normFN = normf(delFN);
-  float etat = m.etat == -1. ? -beta * sqrtf(kn / avmass) : (float)m.etat;
-  float gammat = 2.f * etat * avmass;
-  f3 delFT = -gammat * v_t;
-  float fn = (float)m.muc * normFN;
-  float ft = normf(delFT);
-  if (fn < ft) {
-    float nvt = normf(v_t);
-    f3 tangent{0.f, 0.f, 0.f};
-    if (nvt > 1e-10f) tangent = v_t * (-1.f / nvt);
-    delFT = tangent * fn;
-  }
-  dM = f3{0.f, 0.f, 0.f};
-  if (m.kr != 0.) {
-    f3 wrel = w0 - w1;
-    float nwrel = normf(wrel);
-    float nwtrel = normf(crossf(w0, -l0) - crossf(w1, -l1));
-    if (nwrel > 1e-10f) {
-      float Req = 1.f / (1.f / req0 + 1.f / req1);
-      dM = wrel * (-((float)m.kr * Req * normFN * nwtrel / nwrel));
-    }
-  }
-  f = delFN + delFT;
-}
-
-// ---------------------------------------------------------------------------
-// Integrators.  Sphere angular acceleration T_bf / (c I)
-// (ParticleKinematicsSphere.cpp:47-55).
-struct Kin {
-  d3 x, v, w, wb;
-  quat q;
-};
-
-DEM_HD d3 ang_acc(d3 tbf, double cf, double I) {
-  return d3{tbf.x / (cf * I), tbf.y / (cf * I), tbf.z / (cf * I)};
-}
-
-// SecondOrderLeapFrog SSPRK3 (SecondOrderLeapFrog.cpp:57-66); for a sphere
-// the three stages share one acceleration value.
-DEM_HD void ssprk3(d3& vrot, d3 tbf, double cf, double I, double dt) {
-  d3 K1 = ang_acc(tbf, cf, I);
-  K1 *= dt;
-  d3 K2 = ang_acc(tbf, cf, I);
-  K2 *= dt;
-  d3 K3 = ang_acc(tbf, cf, I);
-  K3 *= dt;
-  vrot += (K1 + K2 + 4. * K3) / 6.;
-}
-
-// Rotation update and translation common to all schemes
-// (ParticleKinematics.cpp:78-103); returns |translational motion|.
-DEM_HD double finish_move(Kin& k, d3 meanW_bf, d3 tm, double dtx) {
-  k.w = to_space(k.q, k.wb);
-  d3 meanW = to_space(k.q, meanW_bf);
-  k.x = k.x + tm;
-  double nO = norm(meanW);
-  quat qdt;
-  if (nO > kEps) {
-    double c = cos(nO * dtx / 2.);
-    double s = sin(nO * dtx / 2.);
-    d3 t = (s * 1. / nO) * meanW;
-    qdt = quat{t, c};
-  } else
-    qdt = quat{d3{0., 0., 0.}, 1.};
-  k.q = qmul(qdt, k.q);
-  return norm(tm);
-}
-
-// ParticleKinematics::Move + SecondOrderLeapFrog::Move (SecondOrderLeapFrog.cpp:44-71)
-DEM_HD double lf_move(Kin& k, d3 F, d3 T, double mass, double cf, double I, double dtv, double dtx) {
-  d3 tbf = to_body(k.q, T);
-  k.v += (F / (mass * cf)) * dtv;
-  d3 tm = k.v * dtx;
-  ssprk3(k.wb, tbf, cf, I, dtv);
-  return finish_move(k, k.wb, tm, dtx);
-}
-// ParticleKinematics::advanceVelocity + SecondOrderLeapFrog::advanceVelocity
-// (ParticleKinematics.cpp:111-128, SecondOrderLeapFrog.cpp:74-95)
-DEM_HD void lf_kick(Kin& k, d3 F, d3 T, double mass, double cf, double I, double dtv) {
-  d3 tbf = to_body(k.q, T);
-  k.v += (F / (mass * cf)) * dtv;
-  ssprk3(k.wb, tbf, cf, I, dtv);
-  k.w = to_space(k.q, k.wb);
-}
-
-struct AB2State {
-  d3 v_nm2, dUdt_nm2, wb_nm2, dWdt_nm2;
-};
-
-// FirstOrderExplicit / SecondOrderExplicit / SecondOrderAdamsBashforth Move
-// (FirstOrderExplicit.cpp:44-61, SecondOrderExplicit.cpp:44-63,
-// SecondOrderAdamsBashforth.cpp:51-101)
-DEM_HD double other_move(int scheme, bool first, Kin& k, AB2State& ab, d3 F, d3 T, double mass, double cf,
-                         double I, double dtv, double dtx) {
-  d3 tbf = to_body(k.q, T);
-  d3 tm, meanW;
-  if (scheme == kFirstOrderExplicit) {
-    tm = k.v * dtx;
-    k.v += (F / (mass * cf)) * dtv;
-    d3 dw = ang_acc(tbf, cf, I);
-    meanW = k.wb;
-    k.wb += dw * dtv;
-  } else if (scheme == kSecondOrderExplicit) {
-    d3 dUdt = F / (mass * cf);
-    tm = k.v * dtx + 0.5 * dUdt * dtx * dtx;
-    k.v += dUdt * dtv;
-    d3 dw = ang_acc(tbf, cf, I);
-    meanW = k.wb + 0.5 * dw * dtx;
-    k.wb += dw * dtv;
-  } else if (first) {
-    d3 dUdt = F / (mass * cf);
-    tm = k.v * dtx;
-    ab.v_nm2 = k.v;
-    k.v += dUdt * dtv;
-    ab.dUdt_nm2 = dUdt;
-    d3 dw = ang_acc(tbf, cf, I);
-    meanW = k.wb;
-    ab.wb_nm2 = k.wb;
-    k.wb += dw * dtv;
-    ab.dWdt_nm2 = dw;
-  } else {
-    d3 dUdt = F / (mass * cf);
-    tm = 0.5 * (3. * k.v - ab.v_nm2) * dtx;
-    ab.v_nm2 = k.v;
-    k.v += 0.5 * (3. * dUdt - ab.dUdt_nm2) * dtv;
-    ab.dUdt_nm2 = dUdt;
-    d3 dw = ang_acc(tbf, cf, I);
-    meanW = 0.5 * (3. * k.wb - ab.wb_nm2);
-    ab.wb_nm2 = k.wb;
-    k.wb += 0.5 * (3. * dw - ab.dWdt_nm2) * dtv;
-    ab.dWdt_nm2 = dw;
-  }
-  return finish_move(k, meanW, tm, dtx);
-}
-
-// ---------------------------------------------------------------------------
-// Loads / stores of one particle's state (storage index s)
-__device__ __forceinline__ d3 ld3(double* const* a, int s) { return d3{a[0][s], a[1][s], a[2][s]}; }
-__device__ __forceinline__ void st3(double* const* a, int s, d3 v) {
-  a[0][s] = v.x;
-  a[1][s] = v.y;
-  a[2][s] = v.z;
-}
-
-__device__ __forceinline__ Kin load_kin(const Particles& P, int s, int cx, int cv) {
-  Kin k;
-  k.x = ld3(P.x[cx], s);
-  k.v = ld3(P.v[cv], s);
-  k.w = ld3(P.w[cv], s);
-  k.wb = ld3(P.wb, s);
-  k.q = quat{d3{P.q[0][s], P.q[1][s], P.q[2][s]}, P.q[3][s]};
-  return k;
-}
-
-// Periodic role swap (LinkedCell.cpp:2004-2310): a particle that drifts into
-// the ghost layer is replaced by its periodic clone, i.e. moved by +-L, and
-// the clone's body-fixed angular velocity is rebuilt from w and q by
-// ParticleKinematics::setAngularVelocity (ParticleKinematics.cpp:197-203).
-// Then the cell key is computed and the arrival slot reserved.
-__device__ __forceinline__ void wrap_and_bin(Kin& k, const Grid& G, const Sorted& S, DevStatus* st, int s, int pid) {
-  bool wrapped = false;
-  double* xs[3] = {&k.x.x, &k.x.y, &k.x.z};
-  int c[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int ri = ref_cell(*xs[a], G, a);
-    if (G.periodic[a]) {
-      if (ri < 1) {
-        *xs[a] = *xs[a] + G.L[a];
-        wrapped = true;
-        ri = ref_cell(*xs[a], G, a);
-      } else if (ri > G.n[a]) {
-        *xs[a] = *xs[a] + (-G.L[a]);
-        wrapped = true;
-        ri = ref_cell(*xs[a], G, a);
-      }
-    }
-    int gi = ri - G.off[a];
-    if (gi < 0 || gi >= G.n[a]) {
-      raise_error(st, kSimulationError, pid, 0);
-      gi = gi < 0 ? 0 : G.n[a] - 1;
-    }
-    c[a] = gi;
-  }
-  if (wrapped) k.wb = to_body(k.q, k.w);
-  int key = (c[1] * G.n[2] + c[2]) * G.n[0] + c[0];  // LinkedCell::getCellNumber order
-  S.key[s] = key;
-  S.slot[s] = atomicAdd(&S.count[key], 1);
-}
 
 // ---------------------------------------------------------------------------
 // First half kick + drift over storage indices (in place): Grains.cpp:353
@@ -636,63 +165,11 @@ __global__ __launch_bounds__(256) void k_scatter(Sorted S, int n) {
 // One thread per cell: order the cell's particles by id (deterministic, and
 // independent of the atomic arrival order), gather the sorted tile arrays and
 // reset the count for the next step.
-constexpr int kCellLocal = 16;
 __global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells, int cx, int n) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
-  int b = min(S.start[c], n), e = min(S.start[c + 1], n);
-  int cnt = e - b;
   S.count[c] = 0;
-  if (cnt == 0) return;
-  if (cnt <= kCellLocal) {
-    int ent[kCellLocal], ids[kCellLocal];
-    for (int i = 0; i < cnt; ++i) {
-      int s = S.tmp_order[b + i];
-      if ((unsigned)s >= (unsigned)n) s = 0;  // only after an interrupted binning
-      int id = P.id[s];
-      int j = i;
-      while (j > 0 && ids[j - 1] > id) {
-        ids[j] = ids[j - 1];
-        ent[j] = ent[j - 1];
-        --j;
-      }
-      ids[j] = id;
-      ent[j] = s;
-    }
-    for (int i = 0; i < cnt; ++i) {
-      int s = ent[i];
-      S.order[b + i] = s;
-      S.sid[b + i] = ids[i];
-      double* pr = S.posr + 4 * (size_t)(b + i);
-      pr[0] = P.x[cx][0][s];
-      pr[1] = P.x[cx][1][s];
-      pr[2] = P.x[cx][2][s];
-      pr[3] = P.rad[s];
-    }
-  } else {
-    // crowded cell: insertion sort in global memory
-    for (int i = 0; i < cnt; ++i) {
-      int s = S.tmp_order[b + i];
-      if ((unsigned)s >= (unsigned)n) s = 0;
-      int id = P.id[s];
-      int j = i;
-      while (j > 0 && S.sid[b + j - 1] > id) {
-        S.sid[b + j] = S.sid[b + j - 1];
-        S.order[b + j] = S.order[b + j - 1];
-        --j;
-      }
-      S.sid[b + j] = id;
-      S.order[b + j] = s;
-    }
-    for (int i = 0; i < cnt; ++i) {
-      int s = S.order[b + i];
-      double* pr = S.posr + 4 * (size_t)(b + i);
-      pr[0] = P.x[cx][0][s];
-      pr[1] = P.x[cx][1][s];
-      pr[2] = P.x[cx][2][s];
-      pr[3] = P.rad[s];
-    }
-  }
+  canon_cell(P, S, min(S.start[c], n), min(S.start[c + 1], n), cx, 0, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -700,23 +177,6 @@ __global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells
 // second half kick + next step's Move (LeapFrog) or forces + Move(dt, dt)
 // (the other schemes).
 enum { kForcesOnly = 0, kForcesKick = 1, kFused = 2 };
-
-struct Self {
-  d3 x, v, w;
-  double r, m, c, req;
-  int mat, id;
-};
-
-__device__ __forceinline__ void record_pair(const StepParams& sp, DevStatus* st, int* pairs, int a, int b) {
-  if (!sp.record_pairs) return;
-  int idx = atomicAdd(&st->npairs, 1);
-  if (idx < sp.pair_cap) {
-    pairs[2 * idx] = a;
-    pairs[2 * idx + 1] = b;
-  } else {
-    st->pair_overflow = 1;
-  }
-}
 
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
@@ -742,134 +202,8 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   // InitializeForce: torsor = weight at the centre (Particle.cpp:1408-1415)
   d3 F{sp.g[0] * wf, sp.g[1] * wf, sp.g[2] * wf};
   d3 T{0., 0., 0.};
-  f3 Fc{0.f, 0.f, 0.f}, Tc{0.f, 0.f, 0.f};
 
-  // own cell, clamped so that a particle flagged outside the grid cannot
-  // index outside the cell table
-  int ci = min(max(ref_cell(me.x.x, G, 0) - G.off[0], 0), G.n[0] - 1);
-  int cj = min(max(ref_cell(me.x.y, G, 1) - G.off[1], 0), G.n[1] - 1);
-  int ck = min(max(ref_cell(me.x.z, G, 2) - G.off[2], 0), G.n[2] - 1);
-
-  for (int dj = -1; dj <= 1; ++dj) {
-    int nj = cj + dj;
-    double sy = 0.;
-    if (G.periodic[1]) {
-      if (nj < 0) { nj += G.n[1]; sy = -G.L[1]; }
-      else if (nj >= G.n[1]) { nj -= G.n[1]; sy = G.L[1]; }
-    } else if (nj < 0 || nj >= G.n[1]) continue;
-    for (int dk = -1; dk <= 1; ++dk) {
-      int nk = ck + dk;
-      double sz = 0.;
-      if (G.periodic[2]) {
-        if (nk < 0) { nk += G.n[2]; sz = -G.L[2]; }
-        else if (nk >= G.n[2]) { nk -= G.n[2]; sz = G.L[2]; }
-      } else if (nk < 0 || nk >= G.n[2]) continue;
-      for (int di = -1; di <= 1; ++di) {
-        int ni = ci + di;
-        double sx = 0.;
-        if (G.periodic[0]) {
-          if (ni < 0) { ni += G.n[0]; sx = -G.L[0]; }
-          else if (ni >= G.n[0]) { ni -= G.n[0]; sx = G.L[0]; }
-        } else if (ni < 0 || ni >= G.n[0]) continue;
-        const int cell = (nj * G.n[2] + nk) * G.n[0] + ni;
-        const int b = S.start[cell], e = min(S.start[cell + 1], sp.n);
-        for (int k2 = b; k2 < e; ++k2) {
-          if (k2 == k) continue;
-          const double* q2 = S.posr + 4 * (size_t)k2;
-          // periodic clone position x + vec (added only on the wrapped axes)
-          d3 xo{sx != 0. ? q2[0] + sx : q2[0], sy != 0. ? q2[1] + sy : q2[1], sz != 0. ? q2[2] + sz : q2[2]};
-          double ro = q2[3];
-          d3 AB = xo - me.x;
-          double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
-          double R = me.r + ro;
-          // conservative reject: s2 > R^2 (1 + 1e-12) implies sqrt(s2) - R > 0
-          if (s2 > R * R * 1.000000000001) continue;
-          double nab = sqrt(s2);
-          const int s2i = S.order[k2];
-          if ((unsigned)s2i >= (unsigned)sp.n) continue;
-          const double co = P.crust[s2i];
-          d3 Pc, ov;
-          double d;
-          int res = sphere_sphere(me.x, me.r, me.c, AB, nab, ro, co, Pc, ov, d);
-          if (res == 0) continue;
-          if (res < 0) {
-            if (!recover_ss(me.x, me.r, me.c, xo, ro, co, Pc, ov, d)) {
-              raise_error(st, kContactError, me.id, S.sid[k2]);
-              continue;
-            }
-            atomicAdd(&st->n_recovered, 1);
-          }
-          if (!(d < 0.)) continue;
-          const int mato = P.mat[s2i];
-          const HookeModel& hm = models[me.mat * kMaxMaterials + mato];
-          d3 vo = ld3(P.v[cv], s2i), wo = ld3(P.w[cv], s2i);
-          double mo = P.mass[s2i], reqo = P.req[s2i];
-          if (MIXED) {
-            // lever arms relative to each centre, in FP32
-            f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - xo);
-            f3 f, dM;
-            hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3of(vo), f3of(wo),
-                    (float)mo, (float)reqo, f3of(ov), f, dM);
-            Fc += f;
-            Tc += crossf(l0, f);
-            if (hm.kr != 0.) Tc += dM;
-          } else {
-            d3 f, dM;
-            hooke64(hm, me.x, me.v, me.w, me.m, me.req, xo, vo, wo, mo, reqo, Pc, ov, f, dM);
-            F += f;
-            T += cross(Pc - me.x, f);  // Torsor::addForce (Torsor.cpp:184-190)
-            if (hm.kr != 0.) T += dM * 1.;
-          }
-          record_pair(sp, st, pairs, me.id, S.sid[k2]);
-        }
-      }
-    }
-  }
-
-  // obstacles (SimpleObstacle::InterAction, SimpleObstacle.cpp:222-271)
-  for (int o = 0; o < sp.nobs; ++o) {
-    const BoxObstacle& B = obs[o];
-    bool probable = fabs(B.aabb_c[0] - me.x.x) <= B.aabb_e[0] + me.r &&
-                    fabs(B.aabb_c[1] - me.x.y) <= B.aabb_e[1] + me.r &&
-                    fabs(B.aabb_c[2] - me.x.z) <= B.aabb_e[2] + me.r;
-    if (!probable) continue;
-    d3 Pc, ov;
-    double d;
-    int res = sphere_box(B, me.x, me.r, me.c, Pc, ov, d);
-    if (res == 0) continue;
-    if (res < 0) {
-      if (!recover_sb(B, me.x, me.r, me.c, Pc, ov, d)) {
-        raise_error(st, kContactError, me.id, B.id);
-        continue;
-      }
-      atomicAdd(&st->n_recovered, 1);
-    }
-    if (!(d < 0.)) continue;
-    const HookeModel& hm = models[me.mat * kMaxMaterials + B.mat];
-    d3 cen{B.centre[0], B.centre[1], B.centre[2]};
-    d3 zero{0., 0., 0.};
-    if (MIXED) {
-      f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - cen);
-      f3 f, dM;
-      hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3{0.f, 0.f, 0.f},
-              f3{0.f, 0.f, 0.f}, (float)kObstacleMass, (float)B.req, f3of(ov), f, dM);
-      Fc += f;
-      Tc += crossf(l0, f);
-      if (hm.kr != 0.) Tc += dM;
-    } else {
-      d3 f, dM;
-      hooke64(hm, me.x, me.v, me.w, me.m, me.req, cen, zero, zero, kObstacleMass, B.req, Pc, ov, f, dM);
-      F += f;
-      T += cross(Pc - me.x, f);
-      if (hm.kr != 0.) T += dM * 1.;
-    }
-    record_pair(sp, st, pairs, me.id, B.id);
-  }
-
-  if (MIXED) {
-    F += d3{(double)Fc.x, (double)Fc.y, (double)Fc.z};
-    T = d3{(double)Tc.x, (double)Tc.y, (double)Tc.z};
-  }
+  sweep_particle<MIXED>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T);
   st3(P.F, s, F);
   st3(P.T, s, T);
   if (MODE == kForcesOnly) return;

@@ -38,6 +38,9 @@ EXPORTS = [
     "grains_gpu_set_state", "grains_gpu_get_masses", "grains_gpu_set_record_pairs", "grains_gpu_get_pairs",
     "grains_gpu_num_particles", "grains_gpu_time", "grains_gpu_set_time", "grains_gpu_grid",
     "grains_gpu_error_ids", "grains_gpu_recoveries", "grains_gpu_last_error",
+    "grains_gpu_batch_create", "grains_gpu_batch_destroy", "grains_gpu_batch_set_block", "grains_gpu_batch_step",
+    "grains_gpu_batch_step_timed", "grains_gpu_batch_autotune", "grains_gpu_batch_num_particles",
+    "grains_gpu_batch_time", "grains_gpu_batch_get_state", "grains_gpu_batch_last_error",
 ]
 
 
@@ -80,6 +83,20 @@ def lib():
         L.grains_gpu_recoveries.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
         L.grains_gpu_last_error.argtypes = [vp]
         L.grains_gpu_last_error.restype = ctypes.c_char_p
+        L.grains_gpu_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.grains_gpu_batch_destroy.argtypes = [vp]
+        L.grains_gpu_batch_destroy.restype = None
+        L.grains_gpu_batch_set_block.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_batch_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.grains_gpu_batch_step_timed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.grains_gpu_batch_autotune.argtypes = [vp, ctypes.c_int, ip, dp]
+        L.grains_gpu_batch_num_particles.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_batch_num_particles.restype = sz
+        L.grains_gpu_batch_time.argtypes = [vp]
+        L.grains_gpu_batch_time.restype = ctypes.c_double
+        L.grains_gpu_batch_get_state.argtypes = [vp, ctypes.c_int] + [dp] * 6
+        L.grains_gpu_batch_last_error.argtypes = [vp]
+        L.grains_gpu_batch_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -251,3 +268,61 @@ class GrainsGPU:
         n = ctypes.c_long()
         self._check(lib().grains_gpu_recoveries(self.h, ctypes.byref(n)))
         return n.value
+
+
+class GrainsBatch:
+    """Independent systems advanced together by one persistent launch.
+
+    ``members`` are GrainsGPU objects that were configured but not
+    initialised; their data is copied into the batch."""
+
+    def __init__(self, members, precision=FP64):
+        L = lib()
+        self.h = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(members))(*[m.h.value for m in members])
+        code = L.grains_gpu_batch_create(arr, len(members), precision, ctypes.byref(self.h))
+        if code:
+            msg = L.grains_gpu_batch_last_error(self.h).decode() if self.h.value else ""
+            raise GrainsError(code, msg)
+        self.nsys = len(members)
+        self.sizes = [int(L.grains_gpu_batch_num_particles(self.h, i)) for i in range(self.nsys)]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value and _lib is not None:
+            _lib.grains_gpu_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, code):
+        if code:
+            raise GrainsError(code, lib().grains_gpu_batch_last_error(self.h).decode())
+
+    @property
+    def n_total(self):
+        return int(lib().grains_gpu_batch_num_particles(self.h, -1))
+
+    def set_block(self, block):
+        self._check(lib().grains_gpu_batch_set_block(self.h, int(block)))
+
+    def step(self, n, chunk=0):
+        self._check(lib().grains_gpu_batch_step(self.h, int(n), int(chunk)))
+
+    def step_timed(self, n, chunk=0):
+        ms = ctypes.c_double()
+        self._check(lib().grains_gpu_batch_step_timed(self.h, int(n), int(chunk), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, trial_steps=20):
+        best = ctypes.c_int()
+        ms = np.zeros(4)
+        self._check(lib().grains_gpu_batch_autotune(self.h, int(trial_steps), ctypes.byref(best), _d(ms)))
+        return best.value, dict(zip((64, 128, 256, 512), ms.tolist()))
+
+    def state(self, i):
+        n = self.sizes[i]
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "w", "F", "T")}
+        out["q"] = np.zeros((n, 4))
+        self._check(lib().grains_gpu_batch_get_state(self.h, int(i), *[_d(out[k]) for k in ("x", "v", "w", "q", "F", "T")]))
+        return out

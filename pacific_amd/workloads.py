@@ -83,6 +83,28 @@ def periodic_gas(n=1_000_000, seed=1, phi=0.3, rmin=0.8e-3, rmax=1.2e-3, density
     }
 
 
+def ensemble(nsys=512, seed=5, nmin=200, nmax=5000, radius=1e-3, density=2500.0, crust=5e-5, kn=1e4, dt=1e-5):
+    """Independent settling boxes: N_i = round(exp(U[ln nmin, ln nmax])) spheres
+    (seed), r = 1 mm, each on a C1-like lattice (spacing 2.5 x 2.5 x 3.75 mm,
+    nx = ny = ceil((N/2)^(1/3)), nz = ceil(N / nx^2)) in its own walled box;
+    per-system mu ~ U[0.1, 0.8] and en ~ U[0.5, 0.95] (seed + 1)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys))).astype(int)
+    rp = np.random.default_rng(seed + 1)
+    mus = rp.uniform(0.1, 0.8, nsys)
+    ens = rp.uniform(0.5, 0.95, nsys)
+    out = []
+    for i, n in enumerate(sizes):
+        nx = int(np.ceil((n / 2.0) ** (1.0 / 3.0)))
+        nz = int(np.ceil(n / (nx * nx)))
+        box = (nx * 2.5e-3, nx * 2.5e-3, nz * 3.75e-3)
+        wl = settling_box(int(n), seed=seed * 1000 + i, radius=radius, density=density, crust=crust, kn=kn,
+                          en=float(ens[i]), muc=float(mus[i]), dt=dt, box=box)
+        wl["name"] = f"ensemble_{i}_{n}"
+        out.append(wl)
+    return out
+
+
 def build(system, wl):
     """Fills a GrainsGPU (or any object with the same methods) from a workload."""
     for (ma, mb, kn, en, muc, kr, etat) in wl["models"]:
