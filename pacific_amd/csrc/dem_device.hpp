@@ -519,6 +519,19 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
   int cj = min(max(ref_cell(me.x.y, G, 1) - G.off[1], 0), G.n[1] - 1);
   int ck = min(max(ref_cell(me.x.z, G, 2) - G.off[2], 0), G.n[2] - 1);
 
+  // The 27 neighbour cells as 9 rows (dj, dk) of up to 3 x-adjacent cells:
+  // cells of one row are consecutive keys, so each row is one contiguous run
+  // of sorted slots.  Cells reached across a periodic x face are handled
+  // apart with their shift.
+  int i0 = ci - 1, i1 = ci + 1;
+  bool wrap_lo = false, wrap_hi = false;
+  if (G.periodic[0]) {
+    if (i0 < 0) { wrap_lo = true; i0 = 0; }
+    if (i1 >= G.n[0]) { wrap_hi = true; i1 = G.n[0] - 1; }
+  } else {
+    i0 = max(i0, 0);
+    i1 = min(i1, G.n[0] - 1);
+  }
   for (int dj = -1; dj <= 1; ++dj) {
     int nj = cj + dj;
     double sy = 0.;
@@ -533,16 +546,27 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
         if (nk < 0) { nk += G.n[2]; sz = -G.L[2]; }
         else if (nk >= G.n[2]) { nk -= G.n[2]; sz = G.L[2]; }
       } else if (nk < 0 || nk >= G.n[2]) continue;
-      for (int di = -1; di <= 1; ++di) {
-        int ni = ci + di;
+      const int row = start_base + (nj * G.n[2] + nk) * G.n[0];
+      // segments: the unshifted run, then the cells across the periodic x faces
+      for (int seg = 0; seg < 3; ++seg) {
+        int c0, c1;
         double sx = 0.;
-        if (G.periodic[0]) {
-          if (ni < 0) { ni += G.n[0]; sx = -G.L[0]; }
-          else if (ni >= G.n[0]) { ni -= G.n[0]; sx = G.L[0]; }
-        } else if (ni < 0 || ni >= G.n[0]) continue;
-        const int cell = (nj * G.n[2] + nk) * G.n[0] + ni;
-        const int b = max(S.start[start_base + cell], slot_lo), e = min(S.start[start_base + cell + 1], slot_hi);
-        for (int k2 = b; k2 < e; ++k2) {
+        if (seg == 0) {
+          c0 = row + i0;
+          c1 = row + i1 + 1;
+        } else if (seg == 1) {
+          if (!wrap_lo) continue;
+          c0 = row + G.n[0] - 1;
+          c1 = c0 + 1;
+          sx = -G.L[0];
+        } else {
+          if (!wrap_hi) continue;
+          c0 = row;
+          c1 = row + 1;
+          sx = G.L[0];
+        }
+        const int b = max(S.start[c0], slot_lo), e = min(S.start[c1], slot_hi);
+      for (int k2 = b; k2 < e; ++k2) {
           if (k2 == k) continue;
           const double* q2 = S.posr + 4 * (size_t)k2;
           // periodic clone position x + vec (added only on the wrapped axes)
@@ -590,7 +614,7 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
             if (hm.kr != 0.) T += dM * 1.;
           }
           record_pair(sp, st, pairs, me.id, S.sid[k2]);
-        }
+      }
       }
     }
   }

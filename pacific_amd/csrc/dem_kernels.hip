@@ -204,8 +204,12 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   d3 T{0., 0., 0.};
 
   sweep_particle<MIXED>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T);
-  st3(P.F, s, F);
-  st3(P.T, s, T);
+  // inside a fused LeapFrog step the torsor is consumed here; it is stored by
+  // the force-only and kick launches that end a batch
+  if (!(MODE == kFused && sp.scheme == kLeapFrog)) {
+    st3(P.F, s, F);
+    st3(P.T, s, T);
+  }
   if (MODE == kForcesOnly) return;
 
   Kin kin;
