@@ -96,6 +96,52 @@ def batched_leg(args, prec):
                                                         "autotune_ms": {str(k): v for k, v in tune.items()}}}
 
 
+def adjoint_leg(args):
+    """Differentiable rollout: 20,000 spheres r~U[0.9,1.1] mm thrown onto a
+    Box plane inclined at 25 degrees, 500 FP64 LeapFrog steps.  Ratio of the
+    time of the checkpointed forward run (state kept every 50 steps) plus the
+    reverse sweep (each segment recomputed from its checkpoint) for the
+    gradient of the final centre-of-mass height, to the time of the plain
+    500-step forward run; wall clock around synchronised calls, both sides."""
+    from pacific_amd import FP64, workloads
+    wl = workloads.inclined_plane()
+    g = build_gpu(wl, FP64)
+    s0 = g.state()
+    n, K = args.adjoint_steps, 50
+    m, _, _ = g.masses()
+    zN = np.zeros((g.n, 3))
+    zN[:, 2] = m / m.sum()
+
+    def reset():
+        g.set_state(x=s0["x"], v=s0["v"], w=s0["w"], q=s0["q"], recompute=True)
+        g.synchronize()
+
+    best = {"plain": 1e30, "fwd": 1e30, "bwd": 1e30}
+    for _ in range(3):
+        reset()
+        t0 = time.perf_counter()
+        g.step(n)
+        g.synchronize()
+        best["plain"] = min(best["plain"], time.perf_counter() - t0)
+        reset()
+        t0 = time.perf_counter()
+        g.adjoint_forward(n, K)
+        t1 = time.perf_counter()
+        grad = g.adjoint_backward(dJ_dx=zN)
+        t2 = time.perf_counter()
+        best["fwd"] = min(best["fwd"], t1 - t0)
+        best["bwd"] = min(best["bwd"], t2 - t1)
+    ratio = (best["fwd"] + best["bwd"]) / best["plain"]
+    log(f"bench: adjoint {wl['name']} {n} steps K={K}: plain {best['plain'] * 1e3:.1f} ms, checkpointed forward "
+        f"{best['fwd'] * 1e3:.1f} ms, reverse {best['bwd'] * 1e3:.1f} ms -> ratio {ratio:.3f}")
+    return {"adjoint_ratio": ratio,
+            "adjoint_config": {"workload": f"{wl['radius'].size} spheres r~U[0.9,1.1] mm onto a Box plane at 25 deg, "
+                                           f"{n} steps dt=1e-5, FP64, checkpoints every {K}",
+                               "plain_ms": best["plain"] * 1e3, "forward_checkpointed_ms": best["fwd"] * 1e3,
+                               "reverse_ms": best["bwd"] * 1e3, "objective": "final centre-of-mass height",
+                               "dJ_dparams": [float(x) for x in grad["params"]]}}
+
+
 def read_traffic(name):
     """HBM bytes per launch of the force sweep from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
@@ -122,6 +168,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost-ratio leg")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--only-adjoint", action="store_true", help="run the adjoint leg alone (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -143,6 +192,10 @@ def main():
         print("bench.py: this build runs the DEM step on one GPU only (no sharding, no collectives); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.only_adjoint:
+        print(json.dumps(adjoint_leg(args)), flush=True)
+        return
 
     from pacific_amd import FP64, MIXED, workloads
     wl = workloads.periodic_gas(args.n, seed=1)
@@ -179,6 +232,8 @@ def main():
 
     if not args.no_batch:
         extra.update(batched_leg(args, prec))
+    if not args.no_adjoint:
+        extra.update(adjoint_leg(args))
 
     traffic = read_traffic(f"traffic_forces_{args.precision}_{args.n}.json")
     out = {

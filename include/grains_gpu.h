@@ -186,6 +186,40 @@ int grains_gpu_batch_get_state(grains_gpu_batch* b, int isys, double* x, double*
                                double* T);
 const char* grains_gpu_batch_last_error(const grains_gpu_batch* b);
 
+/* ---- Discrete adjoint ----------------------------------------------------
+ * Reverse-mode derivative of nsteps SecondOrderLeapFrog steps (spheres, Hooke
+ * model without rolling resistance, FP64) with respect to the initial
+ * positions, velocities, angular velocities and to (kn, en, muc) of one
+ * material pair.  No counterpart in the reference, which has no derivative
+ * code; the step it differentiates is Grains::Simulation's loop body
+ * (Grains.cpp:286-452) with HookeContactForceModel::performForcesCalculus
+ * (HookeContactForceModel.cpp:44-114).
+ *
+ * adjoint_forward advances the system by nsteps steps, keeping the full state
+ * every checkpoint_every steps.  adjoint_backward takes the gradient of a
+ * scalar J with respect to the final x, v, w (id order, 3 per particle, NULL
+ * = zero), rebuilds every segment from its checkpoint, sweeps it in reverse
+ * and writes dJ/dx0, dJ/dv0, dJ/dw0 (NULL skipped) and grad_params =
+ * {dJ/dkn, dJ/den, dJ/dmuc} of the pair (matA, matB); the system is left at
+ * the end state of the forward run.  Memory: 22 doubles per particle per
+ * checkpoint plus 9 per particle per step of one segment. */
+int grains_gpu_adjoint_forward(grains_gpu_system* sys, int nsteps, int checkpoint_every);
+int grains_gpu_adjoint_backward(grains_gpu_system* sys, int matA, int matB, const double* dJ_dx, const double* dJ_dv,
+                                const double* dJ_dw, double* grad_x0, double* grad_v0, double* grad_w0,
+                                double* grad_params);
+
+/* One Hooke contact evaluated on the host by the derivative code of the
+ * adjoint, for checks without a device.  kind 0: body 1 is a sphere, body1 =
+ * {x[3], r, v[3], w[3], m}; kind 1: body 1 is a box obstacle, body1 =
+ * {L[3], centre[3], basis[9] (row-major rotation)}.  body0 = {x[3], r, v[3],
+ * w[3], m} is a sphere; model = {kn, en, etat (-1: from en), muc}.  loads =
+ * {F0, T0, F1, T1} (12; zero without contact).  With cot = {F0b, T0b, F1b,
+ * T1b} (12) grad receives d(F0b.F0 + T0b.T0 + F1b.F1 + T1b.T1) with respect
+ * to body 0's x, v, w and to kn, en, muc (12).  Returns 1 in contact, 0 not,
+ * negative on bad arguments. */
+int grains_contact_eval(int kind, const double* body0, const double* body1, const double* model, const double* cot,
+                        double* loads, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

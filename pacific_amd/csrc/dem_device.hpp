@@ -498,21 +498,23 @@ __device__ __forceinline__ void record_pair(const StepParams& sp, DevStatus* st,
   }
 }
 
-// Contact forces and torques on one particle (sorted slot k) from its 27
-// neighbour cells and the obstacles: LinkedCell::ComputeForces +
-// Particle::InterAction + SimpleObstacle::InterAction + HookeContactForceModel
-// (LinkedCell.cpp:1019-1103, Particle.cpp:477-545, SimpleObstacle.cpp:222-271,
-// HookeContactForceModel.cpp:121-163), gathered on the particle.  F enters
-// holding the weight.  start_base offsets the cell-start table and
-// [slot_lo, slot_hi) bounds the sorted slots and storage indices of the
-// particle's system (one system: 0 and [0, N)).
-template <bool MIXED>
-__device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted& S, const Grid& G,
-                                               const BoxObstacle* __restrict__ obs, int nobs,
-                                               const HookeModel* __restrict__ models, const StepParams& sp,
-                                               DevStatus* st, int* pairs, int start_base, int slot_lo, int slot_hi,
-                                               int cv, int k, const Self& me, d3& F, d3& T) {
-  f3 Fc{0.f, 0.f, 0.f}, Tc{0.f, 0.f, 0.f};
+// Enumerates the contacts (overlap < 0) of one particle (sorted slot k) with
+// its 27 neighbour cells and the obstacles -- LinkedCell::ComputeForces +
+// Particle::InterAction + SimpleObstacle::InterAction (LinkedCell.cpp:1019-1103,
+// Particle.cpp:477-545, SimpleObstacle.cpp:222-271) -- and hands each one to
+// the visitor:
+//   vis.pair(k2, s2i, xo, Pc, ov, d)   neighbour in sorted slot k2, storage
+//                                      s2i, seen at position xo (periodic image)
+//   vis.wall(o, Pc, ov, d)             obstacle o
+// with the contact point Pc, overlap vector ov and overlap distance d < 0 of
+// RigidBodyWithCrust::ClosestPoint (body A = this particle).  start_base
+// offsets the cell-start table; [slot_lo, slot_hi) bounds the sorted slots and
+// storage indices of the particle's system (one system: 0 and [0, N)).
+template <class V>
+__device__ __forceinline__ void for_each_contact(const Particles& P, const Sorted& S, const Grid& G,
+                                                 const BoxObstacle* __restrict__ obs, int nobs, DevStatus* st,
+                                                 int start_base, int slot_lo, int slot_hi, int k, const Self& me,
+                                                 V& vis) {
   // own cell, clamped so that a particle flagged outside the grid cannot
   // index outside the cell table
   int ci = min(max(ref_cell(me.x.x, G, 0) - G.off[0], 0), G.n[0] - 1);
@@ -566,7 +568,7 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
           sx = G.L[0];
         }
         const int b = max(S.start[c0], slot_lo), e = min(S.start[c1], slot_hi);
-      for (int k2 = b; k2 < e; ++k2) {
+        for (int k2 = b; k2 < e; ++k2) {
           if (k2 == k) continue;
           const double* q2 = S.posr + 4 * (size_t)k2;
           // periodic clone position x + vec (added only on the wrapped axes)
@@ -593,28 +595,8 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
             atomicAdd(&st->n_recovered, 1);
           }
           if (!(d < 0.)) continue;
-          const int mato = P.mat[s2i];
-          const HookeModel& hm = models[me.mat * kMaxMaterials + mato];
-          d3 vo = ld3(P.v[cv], s2i), wo = ld3(P.w[cv], s2i);
-          double mo = P.mass[s2i], reqo = P.req[s2i];
-          if (MIXED) {
-            // lever arms relative to each centre, in FP32
-            f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - xo);
-            f3 f, dM;
-            hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3of(vo), f3of(wo),
-                    (float)mo, (float)reqo, f3of(ov), f, dM);
-            Fc += f;
-            Tc += crossf(l0, f);
-            if (hm.kr != 0.) Tc += dM;
-          } else {
-            d3 f, dM;
-            hooke64(hm, me.x, me.v, me.w, me.m, me.req, xo, vo, wo, mo, reqo, Pc, ov, f, dM);
-            F += f;
-            T += cross(Pc - me.x, f);  // Torsor::addForce (Torsor.cpp:184-190)
-            if (hm.kr != 0.) T += dM * 1.;
-          }
-          record_pair(sp, st, pairs, me.id, S.sid[k2]);
-      }
+          vis.pair(k2, s2i, xo, Pc, ov, d);
+        }
       }
     }
   }
@@ -638,6 +620,53 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
       atomicAdd(&st->n_recovered, 1);
     }
     if (!(d < 0.)) continue;
+    vis.wall(o, Pc, ov, d);
+  }
+}
+
+// Hooke-dashpot-Coulomb force and torque gathered on the particle
+// (HookeContactForceModel.cpp:121-163, Torsor.cpp:184-190); F enters holding
+// the weight.  MIXED evaluates the model in FP32 on lever arms.
+template <bool MIXED>
+struct ForceVisitor {
+  const Particles& P;
+  const Sorted& S;
+  const BoxObstacle* obs;
+  const HookeModel* models;
+  const StepParams& sp;
+  DevStatus* st;
+  int* pairs;
+  int cv;
+  const Self& me;
+  d3 F, T;
+  f3 Fc, Tc;
+
+  __device__ __forceinline__ void pair(int k2, int s2i, d3 xo, d3 Pc, d3 ov, double d) {
+    (void)d;
+    const HookeModel& hm = models[me.mat * kMaxMaterials + P.mat[s2i]];
+    d3 vo = ld3(P.v[cv], s2i), wo = ld3(P.w[cv], s2i);
+    double mo = P.mass[s2i], reqo = P.req[s2i];
+    if (MIXED) {
+      f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - xo);
+      f3 f, dM;
+      hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3of(vo), f3of(wo), (float)mo,
+              (float)reqo, f3of(ov), f, dM);
+      Fc += f;
+      Tc += crossf(l0, f);
+      if (hm.kr != 0.) Tc += dM;
+    } else {
+      d3 f, dM;
+      hooke64(hm, me.x, me.v, me.w, me.m, me.req, xo, vo, wo, mo, reqo, Pc, ov, f, dM);
+      F += f;
+      T += cross(Pc - me.x, f);
+      if (hm.kr != 0.) T += dM * 1.;
+    }
+    record_pair(sp, st, pairs, me.id, S.sid[k2]);
+  }
+
+  __device__ __forceinline__ void wall(int o, d3 Pc, d3 ov, double d) {
+    (void)d;
+    const BoxObstacle& B = obs[o];
     const HookeModel& hm = models[me.mat * kMaxMaterials + B.mat];
     d3 cen{B.centre[0], B.centre[1], B.centre[2]};
     d3 zero{0., 0., 0.};
@@ -658,10 +687,21 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
     }
     record_pair(sp, st, pairs, me.id, B.id);
   }
+};
 
+template <bool MIXED>
+__device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted& S, const Grid& G,
+                                               const BoxObstacle* __restrict__ obs, int nobs,
+                                               const HookeModel* __restrict__ models, const StepParams& sp,
+                                               DevStatus* st, int* pairs, int start_base, int slot_lo, int slot_hi,
+                                               int cv, int k, const Self& me, d3& F, d3& T) {
+  ForceVisitor<MIXED> vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
+  for_each_contact(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis);
+  F = vis.F;
+  T = vis.T;
   if (MIXED) {
-    F += d3{(double)Fc.x, (double)Fc.y, (double)Fc.z};
-    T = d3{(double)Tc.x, (double)Tc.y, (double)Tc.z};
+    F += d3{(double)vis.Fc.x, (double)vis.Fc.y, (double)vis.Fc.z};
+    T = d3{(double)vis.Tc.x, (double)vis.Tc.y, (double)vis.Tc.z};
   }
 }
 

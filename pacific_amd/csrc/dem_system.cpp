@@ -55,6 +55,7 @@ void grains_gpu_system::free_device() {
 
 grains_gpu_system::~grains_gpu_system() {
   if (device >= 0) (void)hipSetDevice(device);
+  free_adjoint();
   free_device();
 }
 
@@ -289,9 +290,11 @@ int grains_gpu_system::enqueue_steps_impl(int nsteps) {
     StepParams sp = params();
     HIPCHECK(this, launch_move(P, S, grid, sp, d_status, 0.5 * cfg.dt, cfg.dt, stream));
     if ((e = bin_launch())) return e;
+    if (recording && (e = record_state())) return e;
     for (int i = 1; i < nsteps; ++i) {
       if ((e = forces(kModeFused, 0))) return e;
       if ((e = bin_launch())) return e;
+      if (recording && (e = record_state())) return e;
     }
     if ((e = forces(kModeForcesKick, 0))) return e;
   } else {

@@ -8,6 +8,10 @@
 #include "../../include/grains_gpu.h"
 #include "dem_types.hpp"
 
+namespace dem {
+struct AdjointData;
+}
+
 struct grains_gpu_system {
   grains_gpu_config cfg{};
   int device = 0;
@@ -62,6 +66,13 @@ struct grains_gpu_system {
   Profile* prof = nullptr;
   int mark(std::vector<hipEvent_t>& evs, size_t& n);
   int bin_launch();
+  // adjoint support: checkpoints of a forward run, and a recorder that keeps
+  // binned (x_{n+1}, v_{n+1/2}, w_{n+1/2}) of every LeapFrog step
+  dem::AdjointData* adj = nullptr;
+  bool recording = false;
+  int rec_slot = 0;
+  int record_state();
+  void free_adjoint();
   int enqueue_steps(int nsteps);
   int enqueue_steps_impl(int nsteps);
   int recover();

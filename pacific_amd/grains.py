@@ -41,6 +41,7 @@ EXPORTS = [
     "grains_gpu_batch_create", "grains_gpu_batch_destroy", "grains_gpu_batch_set_block", "grains_gpu_batch_step",
     "grains_gpu_batch_step_timed", "grains_gpu_batch_autotune", "grains_gpu_batch_num_particles",
     "grains_gpu_batch_time", "grains_gpu_batch_get_state", "grains_gpu_batch_last_error",
+    "grains_gpu_adjoint_forward", "grains_gpu_adjoint_backward", "grains_contact_eval",
 ]
 
 
@@ -97,6 +98,9 @@ def lib():
         L.grains_gpu_batch_get_state.argtypes = [vp, ctypes.c_int] + [dp] * 6
         L.grains_gpu_batch_last_error.argtypes = [vp]
         L.grains_gpu_batch_last_error.restype = ctypes.c_char_p
+        L.grains_gpu_adjoint_forward.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.grains_gpu_adjoint_backward.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [dp] * 7
+        L.grains_contact_eval.argtypes = [ctypes.c_int] + [dp] * 6
         _lib = L
     return _lib
 
@@ -110,6 +114,19 @@ def _c64(a, shape=None):
         return None
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a if shape is None else a.reshape(shape)
+
+
+def contact_eval(kind, body0, body1, model, cot=None):
+    """One Hooke contact through the host copy of the adjoint's derivative code
+    (``grains_contact_eval``): returns (in_contact, loads[12], grad[12] or None).
+    Needs the library but no device."""
+    b0, b1, md = _c64(body0), _c64(body1), _c64(model)
+    ct = _c64(cot)
+    loads, grad = np.zeros(12), (np.zeros(12) if cot is not None else None)
+    r = lib().grains_contact_eval(int(kind), _d(b0), _d(b1), _d(md), _d(ct), _d(loads), _d(grad))
+    if r < 0:
+        raise GrainsError(-r if -r in ERRORS else r, "grains_contact_eval: bad arguments")
+    return bool(r), loads, grad
 
 
 class GrainsError(RuntimeError):
@@ -258,6 +275,12 @@ class GrainsGPU:
                                            ctypes.byref(n)))
         return buf[: 2 * n.value].reshape(-1, 2).copy()
 
+    def pair_count(self):
+        """Number of directed contact records of the last force evaluation."""
+        n = ctypes.c_size_t()
+        self._check(lib().grains_gpu_get_pairs(self.h, None, 0, ctypes.byref(n)))
+        return n.value
+
     def grid(self):
         dims = (ctypes.c_int * 3)()
         edge = np.zeros(3)
@@ -268,6 +291,24 @@ class GrainsGPU:
         n = ctypes.c_long()
         self._check(lib().grains_gpu_recoveries(self.h, ctypes.byref(n)))
         return n.value
+
+    def adjoint_forward(self, nsteps, checkpoint_every=50):
+        """nsteps steps, keeping the state every checkpoint_every steps for
+        adjoint_backward."""
+        self._check(lib().grains_gpu_adjoint_forward(self.h, int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, dJ_dx=None, dJ_dv=None, dJ_dw=None, materials=(0, 0)):
+        """Gradients of J with respect to the state at the start of the last
+        adjoint_forward run and to (kn, en, muc) of the material pair:
+        returns dict x0, v0, w0 ([n, 3]) and params (3,)."""
+        n = self.n
+        cot = [_c64(a, (n, 3)) for a in (dJ_dx, dJ_dv, dJ_dw)]
+        out = {k: np.zeros((n, 3)) for k in ("x0", "v0", "w0")}
+        out["params"] = np.zeros(3)
+        self._check(lib().grains_gpu_adjoint_backward(self.h, int(materials[0]), int(materials[1]),
+                                                      *[_d(a) for a in cot], _d(out["x0"]), _d(out["v0"]),
+                                                      _d(out["w0"]), _d(out["params"])))
+        return out
 
 
 class GrainsBatch:

@@ -15,6 +15,14 @@ a checker can be filled from:
                      periodic cube at solid fraction 0.3, Gaussian velocities
                      (sigma 0.1 m/s), no gravity, kn = 1e4, en = 0.8,
                      mu = 0.3, dt = 1e-5 (homogeneous cooling granular gas).
+``ensemble``      -- independent walled boxes of uneven sizes (batched runs).
+``inclined_plane`` -- 20,000 spheres r ~ U[0.9, 1.1] mm (seed 7) in layers
+                     parallel to a Box plane rotated 25 degrees about y,
+                     thrown onto it at 0.2 m/s; gravity, kn = 1e4, en = 0.8,
+                     mu = 0.4, dt = 1e-5 (differentiable rollouts).
+``dense_pack``    -- a small jittered cubic pack of touching spheres in a
+                     walled box (or periodic), every sphere in contact from
+                     the first step (derivative checks).
 """
 import numpy as np
 
@@ -103,6 +111,89 @@ def ensemble(nsys=512, seed=5, nmin=200, nmax=5000, radius=1e-3, density=2500.0,
         wl["name"] = f"ensemble_{i}_{n}"
         out.append(wl)
     return out
+
+
+def _rot_y(deg):
+    a = np.radians(deg)
+    c, s_ = np.cos(a), np.sin(a)
+    return np.array([[c, 0.0, s_], [0.0, 1.0, 0.0], [-s_, 0.0, c]])
+
+
+def inclined_plane(n=20_000, seed=7, angle=25.0, rmin=0.9e-3, rmax=1.1e-3, density=2500.0, crust=5e-5, kn=1e4,
+                   en=0.8, muc=0.4, dt=1e-5, layers=2, pitch=2.3e-3, gap=2e-5, speed=0.2, sigma=0.02):
+    """Spheres on a square lattice of layers parallel to the plane (pitch
+    2.3 mm in the plane and across layers); the first layer floats `gap`
+    above the plane's upper face.  Velocities: `speed` towards the plane plus
+    N(0, sigma) per component (seed + 1)."""
+    rng = np.random.default_rng(seed)
+    radius = rng.uniform(rmin, rmax, n)
+    per_layer = int(np.ceil(n / layers))
+    side = int(np.ceil(np.sqrt(per_layer)))
+    R = _rot_y(angle)
+    e1, e2, nrm = R[:, 0], R[:, 1], R[:, 2]
+    thick = 2e-3
+    plane_len = side * pitch + 10e-3
+    idx = np.arange(n)
+    lay = idx // per_layer
+    rem = idx % per_layer
+    a, b = rem % side, rem // side
+    u = (a - (side - 1) / 2.0) * pitch
+    w = (b - (side - 1) / 2.0) * pitch
+    h = thick / 2 + gap + rmax + lay * pitch
+    x = u[:, None] * e1 + w[:, None] * e2 + (h + radius - rmax)[:, None] * nrm
+    rv = np.random.default_rng(seed + 1)
+    v = -speed * np.tile(nrm, (n, 1)) + rv.normal(0.0, sigma, (n, 3))
+    lo = x.min(axis=0) - 10e-3
+    hi = x.max(axis=0) + 10e-3
+    lo[2] = min(lo[2], -0.5 * plane_len * np.sin(np.radians(angle)) - 10e-3)
+    return {
+        "name": f"inclined_plane_{n}",
+        "origin": tuple(lo), "max": tuple(hi), "periodic": (0, 0, 0), "cellsize_factor": 1.0,
+        "gravity": (0.0, 0.0, -9.81), "dt": dt,
+        "radius": radius, "density": np.full(n, density), "crust": np.full(n, crust),
+        "material": np.zeros(n, np.int32), "x": x, "v": v, "w": np.zeros((n, 3)),
+        "boxes": [((plane_len, plane_len, thick), (0.0, 0.0, 0.0), R.ravel(), crust, 0)],
+        "models": [(0, 0, kn, en, muc, 0.0, -1.0)],
+    }
+
+
+def dense_pack(side=4, seed=3, radius=1e-3, density=2500.0, crust=5e-5, kn=1e4, en=0.8, muc=0.4, dt=1e-5,
+               squeeze=1.5e-5, jitter=4e-6, sigma=0.05, periodic=False, gravity=(0.0, 0.0, -9.81)):
+    """side^3 spheres on a cubic lattice of pitch 2r - squeeze (every
+    neighbour pair overlaps by about `squeeze`), jittered, Gaussian
+    velocities and spins; six walls touching the outer layer, or a periodic
+    cell of the lattice's size."""
+    rng = np.random.default_rng(seed)
+    n = side ** 3
+    pitch = 2 * radius - squeeze
+    i = np.arange(n)
+    a, b, c = i % side, (i // side) % side, i // (side * side)
+    x = np.stack([(a + 0.5) * pitch, (b + 0.5) * pitch, (c + 0.5) * pitch], axis=1)
+    x += rng.uniform(-jitter, jitter, (n, 3))
+    v = rng.normal(0.0, sigma, (n, 3))
+    w = rng.normal(0.0, sigma / radius, (n, 3))
+    Lbox = side * pitch
+    boxes = []
+    if not periodic:
+        t, o = 0.5e-3, 0.5e-3
+        sq = squeeze / 2
+        for ax in range(3):
+            for side_sign in (0, 1):
+                L = [Lbox + 2 * o] * 3
+                L[ax] = t
+                cen = [Lbox / 2] * 3
+                cen[ax] = -t / 2 + sq if side_sign == 0 else Lbox + t / 2 - sq
+                boxes.append((tuple(L), tuple(cen), None, crust, 0))
+    lo = (0.0, 0.0, 0.0) if periodic else (-2e-3, -2e-3, -2e-3)
+    hi = (Lbox, Lbox, Lbox) if periodic else (Lbox + 2e-3, Lbox + 2e-3, Lbox + 2e-3)
+    return {
+        "name": f"dense_pack_{n}",
+        "origin": lo, "max": hi, "periodic": (1, 1, 1) if periodic else (0, 0, 0), "cellsize_factor": 1.0,
+        "gravity": gravity, "dt": dt,
+        "radius": np.full(n, radius), "density": np.full(n, density), "crust": np.full(n, crust),
+        "material": np.zeros(n, np.int32), "x": x, "v": v, "w": w,
+        "boxes": boxes, "models": [(0, 0, kn, en, muc, 0.0, -1.0)],
+    }
 
 
 def build(system, wl):
