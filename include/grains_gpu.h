@@ -147,6 +147,11 @@ int grains_gpu_get_pairs(grains_gpu_system* sys, int* buf, size_t cap, size_t* n
 size_t grains_gpu_num_particles(const grains_gpu_system* sys);
 double grains_gpu_time(const grains_gpu_system* sys);
 void grains_gpu_set_time(grains_gpu_system* sys, double t);
+/* Directed contact records and, of those, contacts on the Coulomb cap
+ * (|FT| = mu |FN|) of the last force evaluation made with recording on
+ * (AppCollision::addToContactsFeatures keeps running contact features,
+ * AppCollision.cpp:159-189). */
+int grains_gpu_contact_counts(grains_gpu_system* sys, size_t* contacts, size_t* sliding);
 /* grid cells per axis (GPU interior cells) */
 int grains_gpu_grid(const grains_gpu_system* sys, int* dims, double* edge);
 /* ids of the bodies named by the last device-side error */

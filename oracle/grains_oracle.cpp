@@ -157,6 +157,13 @@ struct Part {
   V3 weight;
   // Adams-Bashforth n-2 data (SecondOrderAdamsBashforth.cpp:51-101)
   V3 v_nm2, dUdt_nm2, wb_nm2, dWdt_nm2;
+  // convex shape: -1 = sphere of `radius`, else index into System::shapes;
+  // basis of the particle's Transform (rotated by composeLeftByRotation
+  // every move) and the polyhedron support warm start
+  int shape = -1;
+  double B[3][3] = {{1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
+  int curr = 0;
+  double circR = 0.;
 };
 
 struct Obs {
@@ -184,6 +191,8 @@ struct Cell {
 
 struct PairRec { int self, other; };
 
+#include "convex_oracle.hpp"
+
 struct System {
   // domain (App.cpp:35-58) and grid (LinkedCell.cpp:47-140)
   V3 dom_origin, dom_max, dom_size;
@@ -209,6 +218,7 @@ struct System {
   std::vector<int> master_of_id;         // id -> pool index of the master
   std::vector<int> insertion;            // insertion order of ids
   std::vector<Obs> obstacles;
+  std::vector<Shape> shapes;
   bool initialized;
   // diagnostics
   std::vector<PairRec> pairs;
@@ -522,6 +532,199 @@ bool recoverSphereBox(Part const& A, V3 const& xA, Obs const& o, Contact& c) {
 }
 
 // ----------------------------------------------------------------------------
+// General convex pairs: RigidBodyWithCrust::ClosestPoint dispatch and general
+// path (RigidBodyWithCrust.cpp:205-354), ClosestPointSPHEREBOX with the box as
+// body A (:694-716) and ClosestPoint_ErreurHandling (:534-590).
+
+// A rigid body with crust as ClosestPoint sees it
+struct Geo {
+  int type;
+  ConvexRef c;
+  Tf tf;        // Transform without crust
+  double crust, circR;
+  V3 scaling;   // crust shrink factors
+  V3 bext;      // bbox half extents at the identity transform
+};
+
+inline int partType(System const* s, Part const& p) { return p.shape < 0 ? SH_SPHERE : s->shapes[p.shape].type; }
+
+Geo geoOf(System* s, Part& p, V3 const& x) {
+  Geo g;
+  g.type = partType(s, p);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) g.tf.B[i][j] = p.B[i][j];
+  g.tf.o = x;
+  g.crust = p.crust;
+  g.circR = p.radius;
+  if (g.type == SH_SPHERE) {
+    double r = p.shape < 0 ? p.radius : s->shapes[p.shape].radius;
+    g.c = ConvexRef{SH_SPHERE, r, V3(), nullptr, &p.curr};
+    g.bext = V3(r, r, r);
+    for (int a = 0; a < 3; ++a) g.scaling[a] = r < EPSILON ? 1. : (r - p.crust) / r;
+  } else {
+    Shape const& sh = s->shapes[p.shape];
+    g.c = ConvexRef{sh.type, sh.radius, sh.extent, &sh, &p.curr};
+    g.bext = sh.bext;
+    g.scaling = sh.scaling;
+  }
+  return g;
+}
+
+Geo geoOfObs(Obs& o, int* curr) {
+  Geo g;
+  g.type = SH_BOX;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) g.tf.B[i][j] = o.B[i][j];
+  g.tf.o = o.centre;
+  g.crust = o.crust;
+  g.circR = o.circR;
+  g.c = ConvexRef{SH_BOX, 0., o.extent, nullptr, curr};
+  g.bext = o.extent;
+  for (int a = 0; a < 3; ++a) g.scaling[a] = o.extent[a] < EPSILON ? 1. : (o.extent[a] - o.crust) / o.extent[a];
+  return g;
+}
+
+// Transform::composeWithScaling (Transform.cpp:108-112, Matrix.cpp:347-358)
+Tf scaledTf(Geo const& g, V3 const& sc) {
+  Tf t = g.tf;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) t.B[i][j] *= sc[j];
+  return t;
+}
+// getTransformWithCrust(factor, min_scaling) (RigidBodyWithCrust.cpp:770-798)
+Tf scaledTfFactor(Geo const& g, double factor, double min_scaling) {
+  V3 sc;
+  for (int a = 0; a < 3; ++a) {
+    sc[a] = g.bext[a] < EPSILON ? 1. : (g.bext[a] - factor * g.crust) / g.bext[a];
+    if (min_scaling != 1.) sc[a] = std::max(sc[a], min_scaling);
+  }
+  return scaledTf(g, sc);
+}
+
+// Obstacle-like view of a box particle for Box::IntersectionPointSPHERE
+Obs boxView(Geo const& g) {
+  Obs o;
+  o.id = 0;
+  o.extent = g.c.e;
+  o.centre = g.tf.o;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      o.B[i][j] = g.tf.B[i][j];
+      o.Binv[i][j] = g.tf.B[j][i];
+    }
+  for (int i = 0; i < 3; ++i) o.invorigin[i] = dot(-V3(o.Binv[i][0], o.Binv[i][1], o.Binv[i][2]), o.centre);
+  const V3& e = o.extent;
+  o.corners[0] = V3(-e[0], -e[1], -e[2]);
+  o.corners[1] = V3(e[0], -e[1], -e[2]);
+  o.corners[2] = V3(e[0], -e[1], e[2]);
+  o.corners[3] = V3(-e[0], -e[1], e[2]);
+  o.corners[4] = V3(-e[0], e[1], -e[2]);
+  o.corners[5] = V3(e[0], e[1], -e[2]);
+  o.corners[6] = V3(e[0], e[1], e[2]);
+  o.corners[7] = V3(-e[0], e[1], e[2]);
+  o.crust = g.crust;
+  return o;
+}
+
+// ClosestPointSPHEREBOX, box = A, sphere = B (RigidBodyWithCrust.cpp:694-716)
+bool boxSphereA(Geo const& A, Geo const& B, Contact& c) {
+  Obs o = boxView(A);
+  double overlap = 0.;
+  V3 local = applyT(o.Binv, o.invorigin, B.tf.o);
+  V3 cpl;
+  if (!boxSphere(o, local, B.circR, overlap, cpl)) return false;
+  if (overlap < 0.) {
+    if (-overlap >= A.crust + B.crust) return false;
+    c.P = applyT(o.B, o.centre, cpl);
+    V3 AB = B.tf.o - c.P;
+    c.ov = (overlap / Norm(AB)) * AB;
+    c.d = overlap;
+    c.any = true;
+  } else {
+    c.any = false;
+    c.d = 1.e20;
+  }
+  return true;
+}
+// ClosestPointSPHEREBOX, sphere = A, box = B (RigidBodyWithCrust.cpp:661-693)
+bool sphereBoxA(Geo const& A, Geo const& B, Contact& c) {
+  Obs o = boxView(B);
+  double overlap = 0.;
+  V3 local = applyT(o.Binv, o.invorigin, A.tf.o);
+  V3 cpl;
+  if (!boxSphere(o, local, A.circR, overlap, cpl)) return false;
+  if (overlap < 0.) {
+    if (-overlap >= A.crust + B.crust) return false;
+    c.P = applyT(o.B, o.centre, cpl);
+    V3 AB = c.P - A.tf.o;
+    c.ov = (overlap / Norm(AB)) * AB;
+    c.d = overlap;
+    c.any = true;
+  } else {
+    c.any = false;
+    c.d = 1.e20;
+  }
+  return true;
+}
+
+long g_gjk_calls = 0, g_gjk_iters = 0;
+
+// General path (RigidBodyWithCrust.cpp:277-347); false on ContactError
+bool closestPointGJK(Geo const& A, Geo const& B, Contact& c) {
+  V3 gcagcb = A.tf.o - B.tf.o;
+  if (!(Norm(gcagcb) < A.circR + B.circR)) {
+    c.any = false;
+    c.d = 1.e20;
+    return true;
+  }
+  Tf a2w = scaledTf(A, A.scaling), b2w = scaledTf(B, B.scaling);
+  V3 pa, pb;
+  int nb = 0;
+  double distance = gjkClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb);
+  ++g_gjk_calls;
+  g_gjk_iters += nb;
+  if (distance < EPSILON) return false;
+  pa = applyTf(a2w, pa);
+  pb = applyTf(b2w, pb);
+  V3 ba = pa - pb;
+  c.P = pa / 2.0 + pb / 2.0;
+  V3 ov = ba / distance;
+  roundv(ov);
+  ov *= A.crust + B.crust - distance;
+  c.ov = ov;
+  c.d = distance - (A.crust + B.crust);
+  c.any = true;
+  return true;
+}
+
+// ClosestPoint_ErreurHandling with factor 10 (Particle.cpp:505-507,
+// SimpleObstacle.cpp:237-239); false if the shrunk bodies still intersect
+bool closestPointRecoverGJK(Geo const& A, Geo const& B, Contact& c) {
+  Tf a2w = scaledTfFactor(A, 10., 0.5), b2w = scaledTfFactor(B, 10., 0.5);
+  V3 pa, pb;
+  int nb = 0;
+  double distance = gjkClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb);
+  if (distance < EPSILON) return false;
+  pa = applyTf(a2w, pa);
+  pb = applyTf(b2w, pb);
+  V3 ba = pa - pb;
+  ba /= Norm(ba);
+  c.P = pa / 2.0 + pb / 2.0;
+  double imposed = 1. * (A.crust + B.crust);
+  c.ov = imposed * ba;
+  c.d = -imposed;
+  c.any = true;
+  return true;
+}
+
+// ClosestPoint dispatch for pairs with at least one non-sphere body
+bool closestPointConvex(Geo const& A, Geo const& B, Contact& c) {
+  if (A.type == SH_SPHERE && B.type == SH_BOX) return sphereBoxA(A, B, c);
+  if (A.type == SH_BOX && B.type == SH_SPHERE) return boxSphereA(A, B, c);
+  return closestPointGJK(A, B, c);
+}
+
+// ----------------------------------------------------------------------------
 // Hooke force model (HookeContactForceModel.cpp:44-114)
 struct Body {  // what performForcesCalculus reads from a Component
   V3 x, v, w;
@@ -622,12 +825,23 @@ int interactPP(System* s, int ia, int ib) {
   Part& A = s->pool[ia];
   Part& B = s->pool[ib];
   Contact c;
-  if (!sphereSphere(A, A.x, B, B.x, c)) {
-    if (!recoverSphereSphere(A, A.x, B, B.x, c)) {
-      s->err_id0 = A.id; s->err_id1 = B.id;
-      return CONTACT_ERROR;
+  if (A.shape < 0 && B.shape < 0) {
+    if (!sphereSphere(A, A.x, B, B.x, c)) {
+      if (!recoverSphereSphere(A, A.x, B, B.x, c)) {
+        s->err_id0 = A.id; s->err_id1 = B.id;
+        return CONTACT_ERROR;
+      }
+      s->n_recovered++;
     }
-    s->n_recovered++;
+  } else {
+    Geo ga = geoOf(s, A, A.x), gb = geoOf(s, B, B.x);
+    if (!closestPointConvex(ga, gb, c)) {
+      if (!closestPointRecoverGJK(ga, gb, c)) {
+        s->err_id0 = A.id; s->err_id1 = B.id;
+        return CONTACT_ERROR;
+      }
+      s->n_recovered++;
+    }
   }
   contactStats(s, c.any ? c.d : 1.);
   if (c.any && c.d < 0.) {
@@ -653,9 +867,19 @@ int interactPO(System* s, int io, int ip) {
   Contact c;
   c.any = false;
   c.d = 1.e20;
-  if (probable) {
+  if (probable && A.shape < 0) {
     if (!sphereBox(A, A.x, o, c)) {
       if (!recoverSphereBox(A, A.x, o, c)) {
+        s->err_id0 = A.id; s->err_id1 = o.id;
+        return CONTACT_ERROR;
+      }
+      s->n_recovered++;
+    }
+  } else if (probable) {
+    int dummy = 0;
+    Geo ga = geoOf(s, A, A.x), gb = geoOfObs(o, &dummy);
+    if (!closestPointConvex(ga, gb, c)) {
+      if (!closestPointRecoverGJK(ga, gb, c)) {
         s->err_id0 = A.id; s->err_id1 = o.id;
         return CONTACT_ERROR;
       }
@@ -730,6 +954,22 @@ inline V3 angAccSphere(Part const& p, V3 const& tbf, double cf) {
   d[2] = tbf[2] / (cf * p.inertia[5]);
   return d;
 }
+// ParticleKinematics3D::computeAngularAccelerationBodyFixed in predictor mode
+// (ParticleKinematics3D.cpp:56-77; Grains::m_predictor_mode defaults to true)
+inline V3 angAcc3D(Part const& p, V3 const& tbf, double cf, V3 const& om) {
+  const double* I = p.inertia;
+  V3 d;
+  d[0] = tbf[0] / (cf * I[0]) + om[1] * om[2] * (I[3] - I[5]) / I[0];
+  d[1] = tbf[1] / (cf * I[3]) + om[2] * om[0] * (I[5] - I[0]) / I[3];
+  d[2] = tbf[2] / (cf * I[5]) + om[0] * om[1] * (I[0] - I[3]) / I[5];
+  return d;
+}
+// KinematicsBuilderFactory::create: spheres get ParticleKinematicsSphere,
+// other convexes ParticleKinematics3D
+inline V3 angAcc(System const* s, Part const& p, V3 const& tbf, double cf, V3 const& om) {
+  if (p.shape >= 0 && s->shapes[p.shape].type != SH_SPHERE) return angAcc3D(p, tbf, cf, om);
+  return angAccSphere(p, tbf, cf);
+}
 inline double couplingFactor(System const* s, Part const& p) {
   double c = 1.;
   if (s->fluid_density != 0.) c -= s->fluid_density / p.density;  // ParticleKinematics.cpp:485-487
@@ -742,12 +982,12 @@ inline V3 spaceFromBody(Q const& q, V3 const& wb) {  // ParticleKinematics.cpp:7
 
 // SecondOrderLeapFrog SSPRK3 for the body-fixed angular velocity
 // (SecondOrderLeapFrog.cpp:57-66)
-inline void ssprk3(Part const& p, V3 const& tbf, double cf, V3& vrot, double dt) {
-  V3 K1 = angAccSphere(p, tbf, cf);
+inline void ssprk3(System const* s, Part const& p, V3 const& tbf, double cf, V3& vrot, double dt) {
+  V3 K1 = angAcc(s, p, tbf, cf, vrot);
   K1 *= dt;
-  V3 K2 = angAccSphere(p, tbf, cf);  // sphere: independent of omega
+  V3 K2 = angAcc(s, p, tbf, cf, vrot + K1);
   K2 *= dt;
-  V3 K3 = angAccSphere(p, tbf, cf);
+  V3 K3 = angAcc(s, p, tbf, cf, vrot + 0.25 * (K1 + K2));
   K3 *= dt;
   vrot += (K1 + K2 + 4. * K3) / 6.;
 }
@@ -763,13 +1003,13 @@ double kinMove(System* s, Part& p, double dtv, double dtx) {
     case LEAPFROG:  // SecondOrderLeapFrog.cpp:44-71
       v += (p.F / (p.mass * cf)) * dtv;
       transMotion = v * dtx;
-      ssprk3(p, tbf, cf, vrot, dtv);
+      ssprk3(s, p, tbf, cf, vrot, dtv);
       meanW = vrot;
       break;
     case FIRST_ORDER_EXPLICIT: {  // FirstOrderExplicit.cpp:44-61
       transMotion = v * dtx;
       v += (p.F / (p.mass * cf)) * dtv;
-      V3 d = angAccSphere(p, tbf, cf);
+      V3 d = angAcc(s, p, tbf, cf, vrot);
       meanW = vrot;
       vrot += d * dtv;
       break;
@@ -778,7 +1018,7 @@ double kinMove(System* s, Part& p, double dtv, double dtx) {
       V3 dUdt = p.F / (p.mass * cf);
       transMotion = v * dtx + 0.5 * dUdt * dtx * dtx;
       v += dUdt * dtv;
-      V3 d = angAccSphere(p, tbf, cf);
+      V3 d = angAcc(s, p, tbf, cf, vrot);
       meanW = vrot + 0.5 * d * dtx;
       vrot += d * dtv;
       break;
@@ -790,7 +1030,7 @@ double kinMove(System* s, Part& p, double dtv, double dtx) {
         p.v_nm2 = v;
         v += dUdt * dtv;
         p.dUdt_nm2 = dUdt;
-        V3 d = angAccSphere(p, tbf, cf);
+        V3 d = angAcc(s, p, tbf, cf, vrot);
         meanW = vrot;
         p.wb_nm2 = vrot;
         vrot += d * dtv;
@@ -801,7 +1041,7 @@ double kinMove(System* s, Part& p, double dtv, double dtx) {
         p.v_nm2 = v;
         v += 0.5 * (3. * dUdt - p.dUdt_nm2) * dtv;
         p.dUdt_nm2 = dUdt;
-        V3 d = angAccSphere(p, tbf, cf);
+        V3 d = angAcc(s, p, tbf, cf, vrot);
         meanW = 0.5 * (3. * vrot - p.wb_nm2);
         p.wb_nm2 = vrot;
         vrot += 0.5 * (3. * d - p.dWdt_nm2) * dtv;
@@ -823,6 +1063,7 @@ double kinMove(System* s, Part& p, double dtv, double dtx) {
   } else
     qdt = Q(V3(0., 0., 0.), 1.);
   p.q = qmul(qdt, p.q);
+  composeLeftByRotation(p.B, qdt);  // Particle::Rotate -> RigidBody::Rotate (RigidBody.cpp:268-273)
   return Norm(transMotion);
 }
 
@@ -832,7 +1073,7 @@ void kinAdvance(System* s, Part& p, double dtv) {
   double cf = couplingFactor(s, p);
   V3 tbf = torqueBF(p);
   p.v += (p.F / (p.mass * cf)) * dtv;
-  ssprk3(p, tbf, cf, p.wb, dtv);
+  ssprk3(s, p, tbf, cf, p.wb, dtv);
   p.w = spaceFromBody(p.q, p.wb);
 }
 
@@ -896,6 +1137,8 @@ void copyToClone(System* s, Part const& m, Part& c, V3 const& vec) {
   c.x = m.x + vec;
   c.v = m.v;
   c.q = m.q;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) c.B[i][j] = m.B[i][j];
   c.w = m.w;
   c.wb = multToVector3(conj(c.q), vecq(c.w, c.q));
 }
@@ -1103,10 +1346,11 @@ int orc_add_sphere(void* h, double radius, double density, double crust, int mat
   if (basis9) {
     double r[3][3];
     for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) r[i][j] = basis9[3 * i + j];
+      for (int j = 0; j < 3; ++j) r[i][j] = p.B[i][j] = basis9[3 * i + j];
     p.q = quatFromMatrix(r);
   } else
     p.q = Q(V3(0., 0., 0.), 1.);
+  p.circR = radius;
   p.v = V3(v[0], v[1], v[2]);
   p.w = V3(w[0], w[1], w[2]);
   p.wb = multToVector3(conj(p.q), vecq(p.w, p.q));  // ParticleKinematics::setAngularVelocity
@@ -1114,6 +1358,152 @@ int orc_add_sphere(void* h, double radius, double density, double crust, int mat
   s->insertion.push_back(p.id);
   s->master_of_id.push_back((int)s->pool.size() - 1);
   return p.id;
+}
+
+// Convex shape class (ConvexBuilderFactory: <Sphere Radius>, <Box LX LY LZ>,
+// <Polyhedron Name> with a vertex / face file) with its crust thickness;
+// returns the shape index.  type 0 sphere (radius), 1 box (edge lengths L),
+// 2 polyhedron (nv vertices, nf faces given as face_sizes[nf] and the
+// concatenated vertex indices).
+int orc_add_shape(void* h, int type, double radius, const double* L, int nv, const double* verts, int nf,
+                  const int* face_sizes, const int* face_idx, double crust) {
+  System* s = (System*)h;
+  Shape sh;
+  sh.type = type;
+  sh.radius = radius;
+  sh.crust = crust;
+  sh.extent = V3();
+  for (int i = 0; i < 6; ++i) sh.inertia[i] = 0.;
+  if (type == SH_SPHERE) {
+    sh.volume = 4.0 * PI * std::pow(radius, 3.) / 3.0;
+    sh.inertia[0] = sh.inertia[3] = sh.inertia[5] = 8.0 * PI * std::pow(radius, 5.) / 15.0;
+    sh.circR = radius;
+    sh.start_vertex = 0;
+  } else if (type == SH_BOX) {
+    sh.extent = V3(L[0] / 2., L[1] / 2., L[2] / 2.);
+    buildBoxShape(sh);
+  } else if (type == SH_POLY) {
+    for (int i = 0; i < nv; ++i) sh.verts.push_back(V3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]));
+    int off = 0;
+    for (int f = 0; f < nf; ++f) {
+      std::vector<int> face(face_idx + off, face_idx + off + face_sizes[f]);
+      off += face_sizes[f];
+      sh.faces.push_back(face);
+    }
+    buildPolyhedron(sh);
+  } else {
+    return -1;
+  }
+  finishShape(sh);
+  s->shapes.push_back(sh);
+  return (int)s->shapes.size() - 1;
+}
+
+// A particle of a shape class (Particle(DOMNode*), Particle.cpp:130-166):
+// mass = density * volume, inertia = density * shape inertia; basis9 is the
+// row-major rotation of <AngularPosition> (NULL = identity), from which the
+// quaternion follows (Quaternion::setQuaternion(Matrix)).
+int orc_add_particle(void* h, int shape, double density, int material, const double* x, const double* v,
+                     const double* w, const double* basis9) {
+  System* s = (System*)h;
+  if (shape < 0 || shape >= (int)s->shapes.size()) return -1;
+  Shape const& sh = s->shapes[shape];
+  if (sh.type == SH_SPHERE) {
+    int id = orc_add_sphere(h, sh.radius, density, sh.crust, material, x, v, w, basis9);
+    return id;
+  }
+  Part p;
+  p.tag = p.tag_nm1 = p.geoloc = p.geoloc_nm1 = p.cell = p.cell_nm1 = 0;
+  p.id = (int)s->insertion.size() + 1;
+  p.clone = false;
+  p.alive = true;
+  p.shape = shape;
+  p.radius = sh.circR;
+  p.circR = sh.circR;
+  p.crust = sh.crust;
+  p.density = density;
+  p.mat = material;
+  p.mass = density * sh.volume;
+  for (int i = 0; i < 6; ++i) p.inertia[i] = sh.inertia[i] * density;
+  p.req = std::pow((0.75 / PI) * sh.volume, 1. / 3.);
+  p.weight = p.mass * (1. - s->fluid_density / density) * s->gravity;
+  p.x = V3(x[0], x[1], x[2]);
+  if (basis9) {
+    double r[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r[i][j] = p.B[i][j] = basis9[3 * i + j];
+    p.q = quatFromMatrix(r);
+  } else
+    p.q = Q(V3(0., 0., 0.), 1.);
+  p.curr = sh.start_vertex;
+  p.v = V3(v[0], v[1], v[2]);
+  p.w = V3(w[0], w[1], w[2]);
+  p.wb = multToVector3(conj(p.q), vecq(p.w, p.q));
+  s->pool.push_back(p);
+  s->insertion.push_back(p.id);
+  s->master_of_id.push_back((int)s->pool.size() - 1);
+  return p.id;
+}
+
+// Shape properties: out = {volume, circumscribed radius, inertia[6],
+// scaling[3], bbox extent[3]} per unit density
+int orc_shape_info(void* h, int shape, double* out) {
+  System* s = (System*)h;
+  if (shape < 0 || shape >= (int)s->shapes.size()) return -1;
+  Shape const& sh = s->shapes[shape];
+  out[0] = sh.volume;
+  out[1] = sh.circR;
+  for (int i = 0; i < 6; ++i) out[2 + i] = sh.inertia[i];
+  for (int a = 0; a < 3; ++a) out[8 + a] = sh.scaling[a];
+  for (int a = 0; a < 3; ++a) out[11 + a] = sh.bext[a];
+  return 0;
+}
+
+// GJK distance between two shapes placed by (basis9, origin3) each, scaled
+// by their crust shrink factors when `crust` != 0: returns the distance and
+// writes the world witness points pa, pb and the iteration count
+double orc_gjk(void* h, int shapeA, const double* BA, const double* oA, int shapeB, const double* BB,
+               const double* oB, int crust, double* pa, double* pb, int* iters) {
+  System* s = (System*)h;
+  Shape const& sa = s->shapes[shapeA];
+  Shape const& sb = s->shapes[shapeB];
+  int ca = sa.start_vertex, cb = sb.start_vertex;
+  ConvexRef A{sa.type, sa.radius, sa.extent, &sa, &ca}, B{sb.type, sb.radius, sb.extent, &sb, &cb};
+  Tf ta, tb;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      ta.B[i][j] = BA[3 * i + j] * (crust ? sa.scaling[j] : 1.);
+      tb.B[i][j] = BB[3 * i + j] * (crust ? sb.scaling[j] : 1.);
+    }
+  ta.o = V3(oA[0], oA[1], oA[2]);
+  tb.o = V3(oB[0], oB[1], oB[2]);
+  V3 p, q;
+  int it = 0;
+  double d = gjkClosestPoints(A, B, ta, tb, p, q, it);
+  p = applyTf(ta, p);
+  q = applyTf(tb, q);
+  for (int a = 0; a < 3; ++a) {
+    pa[a] = p[a];
+    pb[a] = q[a];
+  }
+  *iters = it;
+  return d;
+}
+
+// Particle bases (id order, 9 per particle, row-major)
+void orc_get_basis(void* h, double* B) {
+  System* s = (System*)h;
+  for (size_t i = 0; i < s->insertion.size(); ++i) {
+    Part const& p = s->pool[s->master_of_id[s->insertion[i]]];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) B[9 * i + 3 * r + c] = p.B[r][c];
+  }
+}
+
+// GJK statistics: calls and total iterations since the library was loaded
+void orc_gjk_stats(long* calls, long* iters) {
+  *calls = g_gjk_calls;
+  *iters = g_gjk_iters;
 }
 
 // n spheres at once (same construction as orc_add_sphere, no basis)

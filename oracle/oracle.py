@@ -54,6 +54,14 @@ def lib():
         L.orc_advance_only.argtypes = [vp, ctypes.c_double]
         L.orc_message.argtypes = [vp]
         L.orc_message.restype = ctypes.c_char_p
+        L.orc_add_shape.argtypes = [vp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, dp, ctypes.c_int, ip, ip,
+                                    ctypes.c_double]
+        L.orc_add_particle.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp]
+        L.orc_shape_info.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_gjk.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, ip]
+        L.orc_gjk.restype = ctypes.c_double
+        L.orc_get_basis.argtypes = [vp, dp]
+        L.orc_gjk_stats.argtypes = [ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
         _lib = L
     return _lib
 
@@ -119,6 +127,61 @@ class Oracle:
         lib().orc_add_box(self.h, _d(np.asarray(L, np.float64)), _d(np.asarray(centre, np.float64)),
                           _d(b), crust, material)
         self.nobs += 1
+
+    SPHERE, BOX, POLYHEDRON = 0, 1, 2
+
+    def add_shape(self, kind, radius=0.0, L=None, verts=None, faces=None, crust=5e-5):
+        """Convex shape class: kind 0 sphere(radius), 1 box(L = edge lengths),
+        2 polyhedron(verts [nv,3], faces = list of vertex-index lists)."""
+        Lv = np.zeros(3) if L is None else np.ascontiguousarray(L, np.float64)
+        nv, nf = 0, 0
+        vv = np.zeros(3)
+        fs = np.zeros(1, np.int32)
+        fi = np.zeros(1, np.int32)
+        if kind == 2:
+            vv = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+            nv, nf = len(vv), len(faces)
+            fs = np.ascontiguousarray([len(f) for f in faces], np.int32)
+            fi = np.ascontiguousarray([i for f in faces for i in f], np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        idx = lib().orc_add_shape(self.h, int(kind), float(radius), _d(Lv), nv, _d(vv), nf, fs.ctypes.data_as(ip),
+                                  fi.ctypes.data_as(ip), float(crust))
+        if idx < 0:
+            raise OracleError(4, None)
+        return idx
+
+    def add_particles(self, shape, density, material, x, v=None, w=None, basis=None):
+        x = np.asarray(x, np.float64).reshape(-1, 3)
+        n = len(x)
+        v = np.zeros((n, 3)) if v is None else np.asarray(v, np.float64).reshape(-1, 3)
+        w = np.zeros((n, 3)) if w is None else np.asarray(w, np.float64).reshape(-1, 3)
+        density = np.broadcast_to(np.asarray(density, np.float64), (n,))
+        material = np.broadcast_to(np.asarray(material, np.int64), (n,))
+        B = None if basis is None else np.asarray(basis, np.float64).reshape(n, 9)
+        for i in range(n):
+            lib().orc_add_particle(self.h, int(shape), float(density[i]), int(material[i]), _d(x[i]), _d(v[i]),
+                                   _d(w[i]), None if B is None else _d(B[i]))
+        self.n += n
+
+    def shape_info(self, shape):
+        out = np.zeros(14)
+        lib().orc_shape_info(self.h, int(shape), _ptr(out))
+        return {"volume": out[0], "circR": out[1], "inertia": out[2:8].copy(), "scaling": out[8:11].copy(),
+                "bbox_extent": out[11:14].copy()}
+
+    def gjk(self, shapeA, BA, oA, shapeB, BB, oB, crust=False):
+        """GJK distance and world witness points between two placed shapes."""
+        pa, pb = np.zeros(3), np.zeros(3)
+        it = ctypes.c_int()
+        d = lib().orc_gjk(self.h, int(shapeA), _d(np.asarray(BA, np.float64).reshape(9)),
+                          _d(np.asarray(oA, np.float64)), int(shapeB), _d(np.asarray(BB, np.float64).reshape(9)),
+                          _d(np.asarray(oB, np.float64)), 1 if crust else 0, _ptr(pa), _ptr(pb), ctypes.byref(it))
+        return d, pa, pb, it.value
+
+    def basis(self):
+        B = np.zeros((self.n, 9))
+        lib().orc_get_basis(self.h, _ptr(B))
+        return B.reshape(self.n, 3, 3)
 
     def _check(self, code):
         if code:

@@ -237,7 +237,8 @@ DEM_HD bool recover_sb(const BoxObstacle& o, d3 xA, double rA, double cA, d3& P,
 // ---------------------------------------------------------------------------
 // Hooke force model, FP64 (HookeContactForceModel.cpp:44-114), p0 = self.
 // Returns the force on p0 and the rolling moment on p0.
-DEM_HD void hooke64(const HookeModel& m, d3 x0, d3 v0, d3 w0, double m0, double req0, d3 x1, d3 v1, d3 w1,
+// Returns true when the Coulomb cap is active.
+DEM_HD bool hooke64(const HookeModel& m, d3 x0, d3 v0, d3 w0, double m0, double req0, d3 x1, d3 v1, d3 w1,
                     double m1, double req1, d3 P, d3 pen, d3& f, d3& dM) {
   d3 normal = pen;
   normal /= norm(normal);
@@ -273,10 +274,11 @@ DEM_HD void hooke64(const HookeModel& m, d3 x0, d3 v0, d3 w0, double m0, double 
   }
   double coef = 1.;
   f = coef * (delFN + delFT);
+  return fn < ft;
 }
 
 // Same model in FP32 on lever arms relative to each body's centre.
-DEM_HD void hooke32(const HookeModel& m, f3 l0, f3 v0, f3 w0, float m0, float req0, f3 l1, f3 v1, f3 w1,
+DEM_HD bool hooke32(const HookeModel& m, f3 l0, f3 v0, f3 w0, float m0, float req0, f3 l1, f3 v1, f3 w1,
                     float m1, float req1, f3 pen, f3& f, f3& dM) {
   const float kn = (float)m.kn, beta = (float)m.beta;
   float npen = normf(pen);
@@ -313,6 +315,7 @@ DEM_HD void hooke32(const HookeModel& m, f3 l0, f3 v0, f3 w0, float m0, float re
     }
   }
   f = delFN + delFT;
+  return fn < ft;
 }
 
 // ---------------------------------------------------------------------------
@@ -487,8 +490,10 @@ struct Self {
   int mat, id;
 };
 
-__device__ __forceinline__ void record_pair(const StepParams& sp, DevStatus* st, int* pairs, int a, int b) {
+__device__ __forceinline__ void record_pair(const StepParams& sp, DevStatus* st, int* pairs, int a, int b,
+                                            bool slip) {
   if (!sp.record_pairs) return;
+  if (slip) atomicAdd(&st->n_slip, 1);
   int idx = atomicAdd(&st->npairs, 1);
   if (idx < sp.pair_cap) {
     pairs[2 * idx] = a;
@@ -646,22 +651,23 @@ struct ForceVisitor {
     const HookeModel& hm = models[me.mat * kMaxMaterials + P.mat[s2i]];
     d3 vo = ld3(P.v[cv], s2i), wo = ld3(P.w[cv], s2i);
     double mo = P.mass[s2i], reqo = P.req[s2i];
+    bool slip;
     if (MIXED) {
       f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - xo);
       f3 f, dM;
-      hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3of(vo), f3of(wo), (float)mo,
-              (float)reqo, f3of(ov), f, dM);
+      slip = hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3of(vo), f3of(wo), (float)mo,
+                     (float)reqo, f3of(ov), f, dM);
       Fc += f;
       Tc += crossf(l0, f);
       if (hm.kr != 0.) Tc += dM;
     } else {
       d3 f, dM;
-      hooke64(hm, me.x, me.v, me.w, me.m, me.req, xo, vo, wo, mo, reqo, Pc, ov, f, dM);
+      slip = hooke64(hm, me.x, me.v, me.w, me.m, me.req, xo, vo, wo, mo, reqo, Pc, ov, f, dM);
       F += f;
       T += cross(Pc - me.x, f);
       if (hm.kr != 0.) T += dM * 1.;
     }
-    record_pair(sp, st, pairs, me.id, S.sid[k2]);
+    record_pair(sp, st, pairs, me.id, S.sid[k2], slip);
   }
 
   __device__ __forceinline__ void wall(int o, d3 Pc, d3 ov, double d) {
@@ -670,22 +676,23 @@ struct ForceVisitor {
     const HookeModel& hm = models[me.mat * kMaxMaterials + B.mat];
     d3 cen{B.centre[0], B.centre[1], B.centre[2]};
     d3 zero{0., 0., 0.};
+    bool slip;
     if (MIXED) {
       f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - cen);
       f3 f, dM;
-      hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3{0.f, 0.f, 0.f},
-              f3{0.f, 0.f, 0.f}, (float)kObstacleMass, (float)B.req, f3of(ov), f, dM);
+      slip = hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3{0.f, 0.f, 0.f},
+                     f3{0.f, 0.f, 0.f}, (float)kObstacleMass, (float)B.req, f3of(ov), f, dM);
       Fc += f;
       Tc += crossf(l0, f);
       if (hm.kr != 0.) Tc += dM;
     } else {
       d3 f, dM;
-      hooke64(hm, me.x, me.v, me.w, me.m, me.req, cen, zero, zero, kObstacleMass, B.req, Pc, ov, f, dM);
+      slip = hooke64(hm, me.x, me.v, me.w, me.m, me.req, cen, zero, zero, kObstacleMass, B.req, Pc, ov, f, dM);
       F += f;
       T += cross(Pc - me.x, f);
       if (hm.kr != 0.) T += dM * 1.;
     }
-    record_pair(sp, st, pairs, me.id, B.id);
+    record_pair(sp, st, pairs, me.id, B.id, slip);
   }
 };
 

@@ -230,8 +230,8 @@ int grains_gpu_system::bin_current() {
 
 int grains_gpu_system::reset_pair_counter() {
   if (!record_pairs) return GRAINS_OK;
-  // npairs and pair_overflow live at fixed offsets of DevStatus
-  HIPCHECK(this, launch_clear(&d_status->npairs, 2, stream));
+  // npairs, pair_overflow and n_slip are consecutive in DevStatus
+  HIPCHECK(this, launch_clear(&d_status->npairs, 3, stream));
   return GRAINS_OK;
 }
 
@@ -687,6 +687,15 @@ int grains_gpu_get_pairs(grains_gpu_system* s, int* buf, size_t cap, size_t* n) 
     size_t m = std::min(np, cap);
     HIPCHECK(s, hipMemcpy(buf, s->d_pairs, sizeof(int) * 2 * m, hipMemcpyDeviceToHost));
   }
+  return GRAINS_OK;
+}
+
+int grains_gpu_contact_counts(grains_gpu_system* s, size_t* contacts, size_t* sliding) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  int e = s->check_status();
+  if (e) return e;
+  if (contacts) *contacts = (size_t)s->last_status.npairs;
+  if (sliding) *sliding = (size_t)s->last_status.n_slip;
   return GRAINS_OK;
 }
 

@@ -89,8 +89,9 @@ struct DevStatus {
   int id0, id1;
   int npairs;        // directed contact records written
   int pair_overflow;
+  int n_slip;        // recorded contacts on the Coulomb cap
   int n_recovered;   // ContactError recoveries
-  int pad[2];
+  int pad;
   double overlap_max;
 };
 

@@ -42,6 +42,7 @@ EXPORTS = [
     "grains_gpu_batch_step_timed", "grains_gpu_batch_autotune", "grains_gpu_batch_num_particles",
     "grains_gpu_batch_time", "grains_gpu_batch_get_state", "grains_gpu_batch_last_error",
     "grains_gpu_adjoint_forward", "grains_gpu_adjoint_backward", "grains_contact_eval",
+    "grains_gpu_contact_counts",
 ]
 
 
@@ -101,6 +102,7 @@ def lib():
         L.grains_gpu_adjoint_forward.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.grains_gpu_adjoint_backward.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [dp] * 7
         L.grains_contact_eval.argtypes = [ctypes.c_int] + [dp] * 6
+        L.grains_gpu_contact_counts.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -275,11 +277,12 @@ class GrainsGPU:
                                            ctypes.byref(n)))
         return buf[: 2 * n.value].reshape(-1, 2).copy()
 
-    def pair_count(self):
-        """Number of directed contact records of the last force evaluation."""
-        n = ctypes.c_size_t()
-        self._check(lib().grains_gpu_get_pairs(self.h, None, 0, ctypes.byref(n)))
-        return n.value
+    def contact_counts(self):
+        """(directed contact records, of which on the Coulomb cap) of the last
+        force evaluation made with pair recording on."""
+        n, ns = ctypes.c_size_t(), ctypes.c_size_t()
+        self._check(lib().grains_gpu_contact_counts(self.h, ctypes.byref(n), ctypes.byref(ns)))
+        return n.value, ns.value
 
     def grid(self):
         dims = (ctypes.c_int * 3)()
