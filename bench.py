@@ -142,6 +142,33 @@ def adjoint_leg(args):
                                "dJ_dparams": [float(x) for x in grad["params"]]}}
 
 
+def polyhedra_leg(args, prec):
+    """200,000 polyhedra (thirds of cubes, tetrahedra, icosahedra) in a
+    silo, random orientations and velocities: GJK narrow phase on
+    crust-shrunk shapes, ParticleKinematics3D, basis composition."""
+    from pacific_amd import workloads
+    wl = workloads.silo(args.poly_n)
+    g = build_gpu(wl, prec)
+    g.step(args.poly_warmup)
+    g.synchronize()
+    ms = g.step_timed(args.poly_steps)
+    value = args.poly_n * args.poly_steps / (ms / 1e3)
+    g.record_pairs(True)
+    g.compute_forces()
+    cbar = len(g.pairs()) / 2.0 / args.poly_n
+    g.record_pairs(False)
+    prof = g.profile_steps(min(args.poly_steps, 100))
+    force_ms = prof["force_ms"] / max(prof["n_force"], 1)
+    log(f"bench: polyhedra silo {args.poly_n}: {value:.4g} particle-steps/s, force sweep {force_ms * 1e3:.1f} us, "
+        f"contacts/particle {cbar:.3f}")
+    return {"polyhedra_value": value,
+            "polyhedra_config": {"workload": f"silo of {args.poly_n} polyhedra (cube 2 mm, tetrahedron and "
+                                             "icosahedron R 1.2 mm), random orientations, sigma_v 0.5 m/s, "
+                                             "GJK, dt 5e-6", "steps": args.poly_steps,
+                                 "ms_per_step": ms / args.poly_steps, "force_sweep_ms": force_ms,
+                                 "contacts_per_particle": cbar}}
+
+
 def read_traffic(name):
     """HBM bytes per launch of the force sweep from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
@@ -169,6 +196,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost-ratio leg")
+    ap.add_argument("--no-poly", action="store_true", help="skip the polyhedra (GJK) leg")
+    ap.add_argument("--poly-n", type=int, default=200_000)
+    ap.add_argument("--poly-steps", type=int, default=400)
+    ap.add_argument("--poly-warmup", type=int, default=100)
+    ap.add_argument("--only-poly", action="store_true", help="run the polyhedra leg alone (profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--only-adjoint", action="store_true", help="run the adjoint leg alone (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -195,6 +227,10 @@ def main():
 
     if args.only_adjoint:
         print(json.dumps(adjoint_leg(args)), flush=True)
+        return
+    if args.only_poly:
+        from pacific_amd import FP64, MIXED
+        print(json.dumps(polyhedra_leg(args, MIXED if args.precision == "mixed" else FP64)), flush=True)
         return
 
     from pacific_amd import FP64, MIXED, workloads
@@ -234,6 +270,8 @@ def main():
         extra.update(batched_leg(args, prec))
     if not args.no_adjoint:
         extra.update(adjoint_leg(args))
+    if not args.no_poly:
+        extra.update(polyhedra_leg(args, prec))
 
     traffic = read_traffic(f"traffic_forces_{args.precision}_{args.n}.json")
     out = {

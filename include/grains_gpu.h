@@ -93,6 +93,34 @@ int grains_gpu_add_spheres(grains_gpu_system* sys, size_t n, const double* radiu
                            const double* crust, const int* material, const double* x, const double* v,
                            const double* w);
 
+/* Non-spherical particles.  A shape class is a convex with its crust
+ * thickness, <Convex CrustThickness><Sphere Radius/> | <Box LX LY LZ/> |
+ * <Polyhedron Name/></Convex> (ConvexBuilderFactory, RigidBodyWithCrust.cpp:
+ * 132-168; Sphere.cpp:61-81, Box.cpp:181-279, Polyhedron.cpp:169-359):
+ * type 0 sphere (radius), 1 box (edge lengths L[3]), 2 polyhedron (nv
+ * vertices, 3 doubles each, and nf faces given by face_sizes[nf] and the
+ * concatenated vertex indices).  *shape_id receives the class index. */
+int grains_gpu_add_shape(grains_gpu_system* sys, int type, double radius, const double* L, int nv,
+                         const double* verts, int nf, const int* face_sizes, const int* face_idx, double crust,
+                         int* shape_id);
+/* Adds n particles of a shape class (ids continue from the current count):
+ * density and material per particle, x, v, w as for spheres, basis = 9
+ * doubles per particle (row-major rotation of <AngularPosition>, NULL =
+ * identity) from which the quaternion follows (Quaternion.cpp:200-255).
+ * Contacts of non-spherical bodies go through GJK on the crust-shrunk
+ * shapes (RigidBodyWithCrust.cpp:277-347); their rotation follows
+ * ParticleKinematics3D (ParticleKinematics3D.cpp:56-77).  SecondOrderLeapFrog
+ * only.  Replaces Particle(DOMNode*) for non-spherical convexes. */
+int grains_gpu_add_particles(grains_gpu_system* sys, int shape, size_t n, const double* density,
+                             const int* material, const double* x, const double* v, const double* w,
+                             const double* basis);
+/* out = {volume, circumscribed radius, inertia xx xy xz yy yz zz, crust
+ * scaling[3], bounding-box half extents[3]}, per unit density */
+int grains_gpu_shape_info(grains_gpu_system* sys, int shape, double* out);
+/* Transform bases of all particles (id order, 9 per particle, row-major) */
+int grains_gpu_get_basis(grains_gpu_system* sys, double* basis);
+int grains_gpu_set_basis(grains_gpu_system* sys, const double* basis);
+
 /* Adds a static box obstacle: edge lengths L[3], centre[3], basis[9] (row
  * major rotation, NULL = identity).  Replaces SimpleObstacle(DOMNode*) with
  * a <Box LX LY LZ> convex (SimpleObstacle.cpp:40-90, Box.cpp:52-60). */

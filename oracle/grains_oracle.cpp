@@ -1500,6 +1500,17 @@ void orc_get_basis(void* h, double* B) {
   }
 }
 
+// Overwrites the particle bases (id order, 9 per particle); clones follow
+// their master on the next step
+void orc_set_basis(void* h, const double* B) {
+  System* s = (System*)h;
+  for (size_t i = 0; i < s->insertion.size(); ++i) {
+    Part& p = s->pool[s->master_of_id[s->insertion[i]]];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) p.B[r][c] = B[9 * i + 3 * r + c];
+  }
+}
+
 // GJK statistics: calls and total iterations since the library was loaded
 void orc_gjk_stats(long* calls, long* iters) {
   *calls = g_gjk_calls;

@@ -61,6 +61,7 @@ def lib():
         L.orc_gjk.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, ip]
         L.orc_gjk.restype = ctypes.c_double
         L.orc_get_basis.argtypes = [vp, dp]
+        L.orc_set_basis.argtypes = [vp, dp]
         L.orc_gjk_stats.argtypes = [ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
         _lib = L
     return _lib
@@ -182,6 +183,9 @@ class Oracle:
         B = np.zeros((self.n, 9))
         lib().orc_get_basis(self.h, _ptr(B))
         return B.reshape(self.n, 3, 3)
+
+    def set_basis(self, B):
+        lib().orc_set_basis(self.h, _d(np.asarray(B, np.float64).reshape(self.n, 9)))
 
     def _check(self, code):
         if code:

@@ -309,6 +309,7 @@ int grains_gpu_adjoint_forward(grains_gpu_system* s, int nsteps, int checkpoint_
   if (!s || !s->initialized || nsteps <= 0 || checkpoint_every <= 0) return GRAINS_ARGUMENT_ERROR;
   if (s->cfg.scheme != GRAINS_LEAPFROG) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates SecondOrderLeapFrog");
   if (s->cfg.precision != GRAINS_FP64) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates the FP64 step");
+  if (s->has_shapes()) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates systems of spheres");
   for (int m = 0; m < kMaxMaterials * kMaxMaterials; ++m)
     if (s->models[m].defined && s->models[m].kr != 0.)
       return afail(s, GRAINS_CONFIG_ERROR, "the adjoint does not differentiate rolling resistance (kr != 0)");

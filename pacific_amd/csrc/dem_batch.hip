@@ -287,6 +287,7 @@ int grains_gpu_batch_create(grains_gpu_system* const* systems, int nsys, int pre
     grains_gpu_system* s = systems[i];
     if (s->initialized) return bfail(b, GRAINS_ARGUMENT_ERROR, "batch members must not be initialised");
     if (s->cfg.scheme != GRAINS_LEAPFROG) return bfail(b, GRAINS_CONFIG_ERROR, "batches run SecondOrderLeapFrog");
+    if (s->has_shapes()) return bfail(b, GRAINS_CONFIG_ERROR, "batch members hold spheres only");
     // model checks and grid of the member (same rules as one system)
     std::vector<int> used(kMaxMaterials, 0);
     for (int m : s->h_mat) used[m] |= 1;

@@ -1,0 +1,330 @@
+// Non-spherical particles on the device: support functions, the GJK distance
+// with Johnson's sub-distance algorithm, the general ClosestPoint path with
+// crust-shrunk transforms and its ContactError handling, and the rigid-body
+// kinematics of ParticleKinematics3D.
+//
+//   RigidBodyWithCrust::ClosestPoint ........ RigidBodyWithCrust.cpp:205-354
+//   ClosestPoint_ErreurHandling ............. RigidBodyWithCrust.cpp:534-590, 770-798
+//   closest_points (Johnson, no Nesterov) ... Convex.cpp:219-435, 536-627
+//   Sphere / Box / Polyhedron::support ...... Sphere.cpp:99-110, Box.cpp:307-316,
+//                                              Polyhedron.cpp:265-293, 431-439
+//   computeAngularAccelerationBodyFixed ..... ParticleKinematics3D.cpp:56-77
+//   Transform::composeLeftByRotation ........ Transform.cpp:140-159
+//
+// The polyhedron support scans all vertices and keeps the first maximum
+// (Polyhedron::support20); the reference hill-climbs from a per-particle warm
+// start.  Both return the maximising vertex; they can differ only among tied
+// maxima, i.e. for a face exactly parallel to the search direction.
+#pragma once
+#include "dem_math.hpp"
+#include "dem_types.hpp"
+
+namespace dem {
+
+constexpr double kEps2 = 1.e-15;  // Basic.hh EPSILON2
+constexpr double kEps3 = 1.e-20;  // Basic.hh EPSILON3
+
+// A rigid body with crust as ClosestPoint sees it
+struct ConvexGeo {
+  int type, nv;
+  const double* verts;
+  double r;       // sphere radius
+  d3 e;           // box half extents
+  double B[9];    // basis rows of the Transform (no crust)
+  d3 o;           // origin
+  d3 scaling;     // crust shrink factors
+  d3 bext;        // bbox half extents at the identity transform
+  double crust, circR;
+};
+
+struct Tf {
+  double B[9];
+  d3 o;
+};
+
+// Transform::operator() (Transform.cpp:312-317)
+DEM_HD d3 tf_apply(const Tf& t, d3 p) {
+  return d3{dot(d3{t.B[0], t.B[1], t.B[2]}, p) + t.o.x, dot(d3{t.B[3], t.B[4], t.B[5]}, p) + t.o.y,
+            dot(d3{t.B[6], t.B[7], t.B[8]}, p) + t.o.z};
+}
+// Vector3 * Matrix (Matrix.cpp:238-241, 527-530)
+DEM_HD d3 tf_vec_basis(d3 v, const Tf& t) {
+  return d3{t.B[0] * v.x + t.B[3] * v.y + t.B[6] * v.z, t.B[1] * v.x + t.B[4] * v.y + t.B[7] * v.z,
+            t.B[2] * v.x + t.B[5] * v.y + t.B[8] * v.z};
+}
+// Transform::composeWithScaling (Matrix.cpp:347-358)
+DEM_HD Tf tf_scaled(const ConvexGeo& g, d3 sc) {
+  Tf t;
+  for (int i = 0; i < 3; ++i) {
+    t.B[3 * i] = g.B[3 * i] * sc.x;
+    t.B[3 * i + 1] = g.B[3 * i + 1] * sc.y;
+    t.B[3 * i + 2] = g.B[3 * i + 2] * sc.z;
+  }
+  t.o = g.o;
+  return t;
+}
+
+DEM_HD d3 support(const ConvexGeo& g, d3 v) {
+  if (g.type == kShapeSphere) {
+    double s = norm(v);
+    if (s > kEps) {
+      double f = g.r / s;
+      return d3{v.x * f, v.y * f, v.z * f};
+    }
+    return d3{0., 0., 0.};
+  }
+  if (g.type == kShapeBox) {
+    double n = norm(v);
+    if (n < kEps) return d3{0., 0., 0.};
+    return d3{v.x < 0. ? -g.e.x : g.e.x, v.y < 0. ? -g.e.y : g.e.y, v.z < 0. ? -g.e.z : g.e.z};
+  }
+  double n = norm(v);
+  if (!(n > kEps)) return d3{0., 0., 0.};
+  const double* p = g.verts;
+  d3 best{p[0], p[1], p[2]};
+  double h = dot(best, v);
+  for (int i = 1; i < g.nv; ++i) {
+    d3 c{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    double d = dot(c, v);
+    if (d > h) {
+      h = d;
+      best = c;
+    }
+  }
+  return best;
+}
+
+// --- Johnson sub-distance algorithm -----------------------------------------
+struct Simplex {
+  d3 y[4], p[4], q[4];
+  double det[16][4];
+  double dp[4][4];
+};
+
+DEM_HD void j_det(Simplex& S, unsigned bits, unsigned last, unsigned last_bit, unsigned all_bits) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if (bits & bit) S.dp[i][last] = S.dp[last][i] = dot(S.y[i], S.y[last]);
+  S.dp[last][last] = dot(S.y[last], S.y[last]);
+  S.det[last_bit][last] = 1.;
+  for (unsigned j = 0, sj = 1; j < 4; ++j, sj <<= 1) {
+    if (!(bits & sj)) continue;
+    unsigned s2 = sj | last_bit;
+    S.det[s2][j] = S.dp[last][last] - S.dp[last][j];
+    S.det[s2][last] = S.dp[j][j] - S.dp[j][last];
+    for (unsigned k = 0, sk = 1; k < j; ++k, sk <<= 1) {
+      if (!(bits & sk)) continue;
+      unsigned s3 = sk | s2;
+      S.det[s3][k] = S.det[s2][j] * (S.dp[j][j] - S.dp[j][k]) + S.det[s2][last] * (S.dp[last][j] - S.dp[last][k]);
+      S.det[s3][j] = S.det[sk | last_bit][k] * (S.dp[k][k] - S.dp[k][j]) +
+                     S.det[sk | last_bit][last] * (S.dp[last][k] - S.dp[last][j]);
+      S.det[s3][last] =
+          S.det[sk | sj][k] * (S.dp[k][k] - S.dp[k][last]) + S.det[sk | sj][j] * (S.dp[j][k] - S.dp[j][last]);
+    }
+  }
+  if (all_bits == 15) {
+    S.det[15][0] = S.det[14][1] * (S.dp[1][1] - S.dp[1][0]) + S.det[14][2] * (S.dp[2][1] - S.dp[2][0]) +
+                   S.det[14][3] * (S.dp[3][1] - S.dp[3][0]);
+    S.det[15][1] = S.det[13][0] * (S.dp[0][0] - S.dp[0][1]) + S.det[13][2] * (S.dp[2][0] - S.dp[2][1]) +
+                   S.det[13][3] * (S.dp[3][0] - S.dp[3][1]);
+    S.det[15][2] = S.det[11][0] * (S.dp[0][0] - S.dp[0][2]) + S.det[11][1] * (S.dp[1][0] - S.dp[1][2]) +
+                   S.det[11][3] * (S.dp[3][0] - S.dp[3][2]);
+    S.det[15][3] = S.det[7][0] * (S.dp[0][0] - S.dp[0][3]) + S.det[7][1] * (S.dp[1][0] - S.dp[1][3]) +
+                   S.det[7][2] * (S.dp[2][0] - S.dp[2][3]);
+  }
+}
+DEM_HD bool j_valid(const Simplex& S, unsigned s, unsigned all_bits) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
+    if (!(all_bits & bit)) continue;
+    if (s & bit) {
+      if (S.det[s][i] <= kEps3) return false;
+    } else if (S.det[s | bit][i] > 0.) {
+      return false;
+    }
+  }
+  return true;
+}
+DEM_HD d3 j_vector(const Simplex& S, unsigned bits) {
+  double sum = 0.;
+  d3 v{0., 0., 0.};
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if (bits & bit) {
+      sum += S.det[bits][i];
+      v += S.det[bits][i] * S.y[i];
+    }
+  v *= 1. / sum;
+  return v;
+}
+DEM_HD void j_points(const Simplex& S, unsigned bits, d3& p1, d3& p2) {
+  double sum = 0.;
+  p1 = d3{0., 0., 0.};
+  p2 = d3{0., 0., 0.};
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if (bits & bit) {
+      sum += S.det[bits][i];
+      p1 += S.det[bits][i] * S.p[i];
+      p2 += S.det[bits][i] * S.q[i];
+    }
+  double s = 1. / sum;
+  p1 *= s;
+  p2 *= s;
+}
+DEM_HD bool j_proper(const Simplex& S, unsigned s) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if ((s & bit) && S.det[s][i] <= kEps3) return false;
+  return true;
+}
+DEM_HD bool j_closest(Simplex& S, unsigned& bits, unsigned last, unsigned last_bit, unsigned all_bits, d3& v) {
+  j_det(S, bits, last, last_bit, all_bits);
+  for (unsigned s = bits; s; --s) {
+    if ((s & bits) == s && j_valid(S, s | last_bit, all_bits)) {
+      bits = s | last_bit;
+      v = j_vector(S, bits);
+      return true;
+    }
+  }
+  if (j_valid(S, last_bit, all_bits)) {
+    bits = last_bit;
+    v = S.y[last];
+    return true;
+  }
+  double min_dist2 = INFINITY;
+  for (unsigned s = all_bits; s; --s) {
+    if ((s & all_bits) == s && j_proper(S, s)) {
+      d3 u = j_vector(S, s);
+      double d2 = dot(u, u);
+      if (d2 < min_dist2) {
+        min_dist2 = d2;
+        bits = s;
+        v = u;
+      }
+    }
+  }
+  return false;
+}
+DEM_HD bool j_degenerate(const Simplex& S, unsigned all_bits, d3 w) {
+  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
+    if ((all_bits & bit) && S.y[i].x == w.x && S.y[i].y == w.y && S.y[i].z == w.z) return true;
+  return false;
+}
+
+// closest_points with relative tolerance 1e-10; witness points in the local
+// frames; returns the distance
+DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, d3& pa, d3& pb,
+                          int& iters) {
+  Simplex S;
+  for (int i = 0; i < 16; ++i)
+    for (int j = 0; j < 4; ++j) S.det[i][j] = 0.;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) S.dp[i][j] = 0.;
+  unsigned bits = 0, last = 0, last_bit = 0, all_bits = 0;
+  const double relError = kEps, absError = 1.e-4 * relError;
+  d3 zero{0., 0., 0.};
+  d3 v = tf_apply(a2w, support(A, zero)) - tf_apply(b2w, support(B, zero));
+  d3 w = v;
+  double dist = norm(v);
+  int it = 0;
+  while (bits < 15 && dist > kEps2 && it < 1000) {
+    ++it;
+    last = 0;
+    last_bit = 1;
+    while (bits & last_bit) {
+      ++last;
+      last_bit <<= 1;
+    }
+    d3 d = v;
+    S.p[last] = support(A, tf_vec_basis(-d, a2w));
+    S.q[last] = support(B, tf_vec_basis(d, b2w));
+    w = tf_apply(a2w, S.p[last]) - tf_apply(b2w, S.q[last]);
+    double mu = dist - dot(v, w) / dist;
+    if (mu < dist * relError || mu < absError) break;
+    if (j_degenerate(S, all_bits, w)) break;
+    S.y[last] = w;
+    all_bits = bits | last_bit;
+    if (!j_closest(S, bits, last, last_bit, all_bits, v)) break;
+    dist = norm(v);
+  }
+  j_points(S, bits, pa, pb);
+  iters = it;
+  return dist;
+}
+
+// General ClosestPoint path.  Returns 1 with the contact features (the
+// caller tests d < 0), 0 when the circumscribed spheres do not meet, -1 on
+// ContactError (the crust-shrunk bodies intersect).
+DEM_HD int closest_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d) {
+  d3 gcagcb = A.o - B.o;
+  if (!(norm(gcagcb) < A.circR + B.circR)) return 0;
+  Tf a2w = tf_scaled(A, A.scaling), b2w = tf_scaled(B, B.scaling);
+  d3 pa, pb;
+  int it;
+  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, it);
+  if (distance < kEps) return -1;
+  pa = tf_apply(a2w, pa);
+  pb = tf_apply(b2w, pb);
+  d3 ba = pa - pb;
+  P = pa / 2.0 + pb / 2.0;
+  d3 o = ba / distance;
+  round_small(o);
+  o *= A.crust + B.crust - distance;
+  ov = o;
+  d = distance - (A.crust + B.crust);
+  return 1;
+}
+
+// ClosestPoint_ErreurHandling(factor 10): false if still intersecting
+DEM_HD bool recover_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d) {
+  d3 sa, sb;
+  double* as[3] = {&sa.x, &sa.y, &sa.z};
+  double* bs[3] = {&sb.x, &sb.y, &sb.z};
+  const double ae[3] = {A.bext.x, A.bext.y, A.bext.z}, be[3] = {B.bext.x, B.bext.y, B.bext.z};
+  for (int a = 0; a < 3; ++a) {
+    double x = ae[a] < kEps ? 1. : (ae[a] - 10. * A.crust) / ae[a];
+    *as[a] = x > 0.5 ? x : 0.5;
+    double y = be[a] < kEps ? 1. : (be[a] - 10. * B.crust) / be[a];
+    *bs[a] = y > 0.5 ? y : 0.5;
+  }
+  Tf a2w = tf_scaled(A, sa), b2w = tf_scaled(B, sb);
+  d3 pa, pb;
+  int it;
+  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, it);
+  if (distance < kEps) return false;
+  pa = tf_apply(a2w, pa);
+  pb = tf_apply(b2w, pb);
+  d3 ba = pa - pb;
+  ba /= norm(ba);
+  P = pa / 2.0 + pb / 2.0;
+  double imposed = 1. * (A.crust + B.crust);
+  ov = imposed * ba;
+  d = -imposed;
+  return true;
+}
+
+// --- kinematics of non-spherical bodies --------------------------------------
+DEM_HD d3 ang_acc3(d3 tbf, double cf, d3 I, d3 om) {
+  return d3{tbf.x / (cf * I.x) + om.y * om.z * (I.y - I.z) / I.x, tbf.y / (cf * I.y) + om.z * om.x * (I.z - I.x) / I.y,
+            tbf.z / (cf * I.z) + om.x * om.y * (I.x - I.y) / I.z};
+}
+DEM_HD void ssprk3_3(d3& vrot, d3 tbf, double cf, d3 I, double dt) {
+  d3 K1 = ang_acc3(tbf, cf, I, vrot);
+  K1 *= dt;
+  d3 K2 = ang_acc3(tbf, cf, I, vrot + K1);
+  K2 *= dt;
+  d3 K3 = ang_acc3(tbf, cf, I, vrot + 0.25 * (K1 + K2));
+  K3 *= dt;
+  vrot += (K1 + K2 + 4. * K3) / 6.;
+}
+// basis <- R(q) basis
+DEM_HD void compose_left(double* B, quat q) {
+  const double qx = q.v.x, qy = q.v.y, qz = q.v.z, qw = q.w;
+  for (int i = 0; i < 3; ++i) {
+    double px = qy * B[6 + i] - qz * B[3 + i] + qw * B[i];
+    double py = qz * B[i] - qx * B[6 + i] + qw * B[3 + i];
+    double pz = qx * B[3 + i] - qy * B[i] + qw * B[6 + i];
+    double pw = -qx * B[i] - qy * B[3 + i] - qz * B[6 + i];
+    B[i] = qy * pz - qz * py - pw * qx + qw * px;
+    B[3 + i] = qz * px - qx * pz - pw * qy + qw * py;
+    B[6 + i] = qx * py - qy * px - pw * qz + qw * pz;
+  }
+}
+
+}  // namespace dem

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "dem_convex.hpp"
 #include "dem_math.hpp"
 #include "dem_types.hpp"
 
@@ -67,8 +68,10 @@ DEM_HD int sphere_sphere(d3 xA, double rA, double cA, d3 AB, double nab, double 
   return 1;
 }
 
-DEM_HD d3 corner_of(const BoxObstacle& o, int n) {
-  return d3{o.corners[3 * n], o.corners[3 * n + 1], o.corners[3 * n + 2]};
+// Box corner n in the numbering of Box::setCornersFaces (Box.cpp:92-110)
+DEM_HD d3 corner_of(const double* e, int n) {
+  const bool px = n == 1 || n == 2 || n == 5 || n == 6, py = n >= 4, pz = n == 2 || n == 3 || n == 6 || n == 7;
+  return d3{px ? e[0] : -e[0], py ? e[1] : -e[1], pz ? e[2] : -e[2]};
 }
 DEM_HD double comp(d3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 DEM_HD void set_comp(d3& a, int i, double v) {
@@ -76,8 +79,8 @@ DEM_HD void set_comp(d3& a, int i, double v) {
 }
 
 // Box::ContactCornerSPHERE (Box.cpp:725-744)
-DEM_HD void box_corner(const BoxObstacle& o, d3 C, double R, int n, double& overlap, d3& cp) {
-  d3 dist = C - corner_of(o, n);
+DEM_HD void box_corner(const double* e, d3 C, double R, int n, double& overlap, d3& cp) {
+  d3 dist = C - corner_of(e, n);
   double nd = norm(dist);
   if (nd < R) {
     overlap = nd - R;
@@ -85,9 +88,9 @@ DEM_HD void box_corner(const BoxObstacle& o, d3 C, double R, int n, double& over
   }
 }
 // Box::ContactEdgeSPHERE (Box.cpp:750-769)
-DEM_HD void box_edge(const BoxObstacle& o, d3 C, double R, int n, int dir, double& overlap, d3& cp) {
+DEM_HD void box_edge(const double* e, d3 C, double R, int n, int dir, double& overlap, d3& cp) {
   d3 proj = C;
-  d3 cn = corner_of(o, n);
+  d3 cn = corner_of(e, n);
   set_comp(proj, dir, comp(cn, dir));
   d3 dist = proj - cn;
   double nd = norm(dist);
@@ -98,23 +101,23 @@ DEM_HD void box_edge(const BoxObstacle& o, d3 C, double R, int n, int dir, doubl
 }
 // Box::IntersectionPointSPHERE (Box.cpp:466-716); false if the centre lies in
 // the box (a ContactError in the reference)
-DEM_HD bool box_sphere(const BoxObstacle& o, d3 C, double R, double& overlap, d3& cp) {
-  const double ex = o.extent[0], ey = o.extent[1], ez = o.extent[2];
+DEM_HD bool box_sphere(const double* e, d3 C, double R, double& overlap, d3& cp) {
+  const double ex = e[0], ey = e[1], ez = e[2];
   const double gx = C.x, gy = C.y, gz = C.z;
   overlap = 1.;
   cp = d3{0., 0., 0.};
   if (gx > ex) {
     if (gy > ey) {
-      if (gz > ez) box_corner(o, C, R, 6, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 5, overlap, cp);
-      else box_edge(o, C, R, 5, 2, overlap, cp);
+      if (gz > ez) box_corner(e, C, R, 6, overlap, cp);
+      else if (gz < -ez) box_corner(e, C, R, 5, overlap, cp);
+      else box_edge(e, C, R, 5, 2, overlap, cp);
     } else if (gy < -ey) {
-      if (gz > ez) box_corner(o, C, R, 2, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 1, overlap, cp);
-      else box_edge(o, C, R, 1, 2, overlap, cp);
+      if (gz > ez) box_corner(e, C, R, 2, overlap, cp);
+      else if (gz < -ez) box_corner(e, C, R, 1, overlap, cp);
+      else box_edge(e, C, R, 1, 2, overlap, cp);
     } else {
-      if (gz > ez) box_edge(o, C, R, 6, 1, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 1, 1, overlap, cp);
+      if (gz > ez) box_edge(e, C, R, 6, 1, overlap, cp);
+      else if (gz < -ez) box_edge(e, C, R, 1, 1, overlap, cp);
       else {
         double nd = gx - ex;
         if (nd < R) { overlap = nd - R; cp = d3{ex + 0.5 * overlap, C.y, C.z}; }
@@ -122,16 +125,16 @@ DEM_HD bool box_sphere(const BoxObstacle& o, d3 C, double R, double& overlap, d3
     }
   } else if (gx < -ex) {
     if (gy > ey) {
-      if (gz > ez) box_corner(o, C, R, 7, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 4, overlap, cp);
-      else box_edge(o, C, R, 4, 2, overlap, cp);
+      if (gz > ez) box_corner(e, C, R, 7, overlap, cp);
+      else if (gz < -ez) box_corner(e, C, R, 4, overlap, cp);
+      else box_edge(e, C, R, 4, 2, overlap, cp);
     } else if (gy < -ey) {
-      if (gz > ez) box_corner(o, C, R, 3, overlap, cp);
-      else if (gz < -ez) box_corner(o, C, R, 0, overlap, cp);
-      else box_edge(o, C, R, 0, 2, overlap, cp);
+      if (gz > ez) box_corner(e, C, R, 3, overlap, cp);
+      else if (gz < -ez) box_corner(e, C, R, 0, overlap, cp);
+      else box_edge(e, C, R, 0, 2, overlap, cp);
     } else {
-      if (gz > ez) box_edge(o, C, R, 3, 1, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 0, 1, overlap, cp);
+      if (gz > ez) box_edge(e, C, R, 3, 1, overlap, cp);
+      else if (gz < -ez) box_edge(e, C, R, 0, 1, overlap, cp);
       else {
         double nd = -ex - gx;
         if (nd < R) { overlap = nd - R; cp = d3{-ex - 0.5 * overlap, C.y, C.z}; }
@@ -139,15 +142,15 @@ DEM_HD bool box_sphere(const BoxObstacle& o, d3 C, double R, double& overlap, d3
     }
   } else {
     if (gy > ey) {
-      if (gz > ez) box_edge(o, C, R, 6, 0, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 4, 0, overlap, cp);
+      if (gz > ez) box_edge(e, C, R, 6, 0, overlap, cp);
+      else if (gz < -ez) box_edge(e, C, R, 4, 0, overlap, cp);
       else {
         double nd = gy - ey;
         if (nd < R) { overlap = nd - R; cp = d3{C.x, ey + 0.5 * overlap, C.z}; }
       }
     } else if (gy < -ey) {
-      if (gz > ez) box_edge(o, C, R, 3, 0, overlap, cp);
-      else if (gz < -ez) box_edge(o, C, R, 0, 0, overlap, cp);
+      if (gz > ez) box_edge(e, C, R, 3, 0, overlap, cp);
+      else if (gz < -ez) box_edge(e, C, R, 0, 0, overlap, cp);
       else {
         double nd = -ey - gy;
         if (nd < R) { overlap = nd - R; cp = d3{C.x, -ey - 0.5 * overlap, C.z}; }
@@ -178,7 +181,7 @@ DEM_HD int sphere_box(const BoxObstacle& o, d3 xA, double rA, double cA, d3& P, 
   double overlap = 0.;
   d3 local = apply_tf(o.Binv, o.invorigin, xA);
   d3 cpl;
-  if (!box_sphere(o, local, rA, overlap, cpl)) return -1;
+  if (!box_sphere(o.extent, local, rA, overlap, cpl)) return -1;
   if (!(overlap < 0.)) return 0;
   if (-overlap >= cA + o.crust) return -1;
   P = apply_tf(o.B, o.centre, cpl);
@@ -232,6 +235,39 @@ DEM_HD bool recover_sb(const BoxObstacle& o, d3 xA, double rA, double cA, d3& P,
   ov = imposed * ba;
   d = -imposed;
   return true;
+}
+
+// ClosestPointSPHEREBOX against a box particle: the sphere is body A
+// (RigidBodyWithCrust.cpp:661-693) or the box is (:694-716).  The box's
+// inverse transform is its transposed basis (Transform.cpp:71-77).
+DEM_HD int sphere_box_particle(const ConvexGeo& bx, const ConvexGeo& sp, bool sphere_first, d3& P, d3& ov,
+                               double& d) {
+  double Binv[9], io[3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Binv[3 * i + j] = bx.B[3 * j + i];
+  for (int i = 0; i < 3; ++i) io[i] = dot(-d3{Binv[3 * i], Binv[3 * i + 1], Binv[3 * i + 2]}, bx.o);
+  const double e[3] = {bx.e.x, bx.e.y, bx.e.z};
+  const double oc[3] = {bx.o.x, bx.o.y, bx.o.z};
+  double overlap = 0.;
+  d3 local = apply_tf(Binv, io, sp.o);
+  d3 cpl;
+  if (!box_sphere(e, local, sp.circR, overlap, cpl)) return -1;
+  if (!(overlap < 0.)) return 0;
+  if (-overlap >= sp.crust + bx.crust) return -1;
+  P = apply_tf(bx.B, oc, cpl);
+  d3 AB = sphere_first ? P - sp.o : sp.o - P;
+  ov = (overlap / norm(AB)) * AB;
+  d = overlap;
+  return 1;
+}
+
+// RigidBodyWithCrust::ClosestPoint dispatch for a pair with at least one
+// non-spherical body.  1: features set (contact if d < 0); 0: no contact;
+// -1: ContactError.
+DEM_HD int closest_convex(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d) {
+  if (A.type == kShapeSphere && B.type == kShapeBox) return sphere_box_particle(B, A, true, P, ov, d);
+  if (A.type == kShapeBox && B.type == kShapeSphere) return sphere_box_particle(A, B, false, P, ov, d);
+  return closest_gjk(A, B, P, ov, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,7 +380,7 @@ DEM_HD void ssprk3(d3& vrot, d3 tbf, double cf, double I, double dt) {
 
 // Rotation update and translation common to all schemes
 // (ParticleKinematics.cpp:78-103); returns |translational motion|.
-DEM_HD double finish_move(Kin& k, d3 meanW_bf, d3 tm, double dtx) {
+DEM_HD double finish_move(Kin& k, d3 meanW_bf, d3 tm, double dtx, quat* qdt_out = nullptr) {
   k.w = to_space(k.q, k.wb);
   d3 meanW = to_space(k.q, meanW_bf);
   k.x = k.x + tm;
@@ -358,6 +394,7 @@ DEM_HD double finish_move(Kin& k, d3 meanW_bf, d3 tm, double dtx) {
   } else
     qdt = quat{d3{0., 0., 0.}, 1.};
   k.q = qmul(qdt, k.q);
+  if (qdt_out) *qdt_out = qdt;
   return norm(tm);
 }
 
@@ -375,6 +412,30 @@ DEM_HD void lf_kick(Kin& k, d3 F, d3 T, double mass, double cf, double I, double
   d3 tbf = to_body(k.q, T);
   k.v += (F / (mass * cf)) * dtv;
   ssprk3(k.wb, tbf, cf, I, dtv);
+  k.w = to_space(k.q, k.wb);
+}
+
+// The same for a body of a system with non-spherical particles: bodies
+// other than spheres use ParticleKinematics3D (principal moments I3), and
+// every body's basis is rotated by the step's quaternion increment
+// (Particle::Rotate -> Transform::composeLeftByRotation).
+DEM_HD double lf_move_body(Kin& k, bool gyro, d3 F, d3 T, double mass, double cf, double I, d3 I3, double dtv,
+                           double dtx, double* B) {
+  d3 tbf = to_body(k.q, T);
+  k.v += (F / (mass * cf)) * dtv;
+  d3 tm = k.v * dtx;
+  if (gyro) ssprk3_3(k.wb, tbf, cf, I3, dtv);
+  else ssprk3(k.wb, tbf, cf, I, dtv);
+  quat qdt;
+  double depl = finish_move(k, k.wb, tm, dtx, &qdt);
+  compose_left(B, qdt);
+  return depl;
+}
+DEM_HD void lf_kick_body(Kin& k, bool gyro, d3 F, d3 T, double mass, double cf, double I, d3 I3, double dtv) {
+  d3 tbf = to_body(k.q, T);
+  k.v += (F / (mass * cf)) * dtv;
+  if (gyro) ssprk3_3(k.wb, tbf, cf, I3, dtv);
+  else ssprk3(k.wb, tbf, cf, I, dtv);
   k.w = to_space(k.q, k.wb);
 }
 
@@ -515,11 +576,64 @@ __device__ __forceinline__ void record_pair(const StepParams& sp, DevStatus* st,
 // RigidBodyWithCrust::ClosestPoint (body A = this particle).  start_base
 // offsets the cell-start table; [slot_lo, slot_hi) bounds the sorted slots and
 // storage indices of the particle's system (one system: 0 and [0, N)).
-template <class V>
+// With SHAPES the system holds non-spherical particles: mg describes this
+// particle's body (its shape, basis and crust) and cx selects the basis
+// buffer of the neighbours.
+__device__ __forceinline__ ConvexGeo convex_of(const Particles& P, int s, d3 x, int cx) {
+  ConvexGeo g;
+  const int sh = P.shape[s];
+  for (int j = 0; j < 9; ++j) g.B[j] = P.B[cx][j][s];
+  g.o = x;
+  if (sh < 0) {
+    const double r = P.rad[s], c = P.crust[s];
+    g.type = kShapeSphere;
+    g.nv = 0;
+    g.verts = nullptr;
+    g.r = r;
+    g.e = d3{0., 0., 0.};
+    const double f = r < kEps ? 1. : (r - c) / r;
+    g.scaling = d3{f, f, f};
+    g.bext = d3{r, r, r};
+    g.crust = c;
+    g.circR = r;
+  } else {
+    const ShapeDesc& d = P.shapes[sh];
+    g.type = d.type;
+    g.nv = d.nv;
+    g.verts = P.verts + 3 * (size_t)d.vert_off;
+    g.r = d.radius;
+    g.e = d3{d.extent[0], d.extent[1], d.extent[2]};
+    g.scaling = d3{d.scaling[0], d.scaling[1], d.scaling[2]};
+    g.bext = d3{d.bext[0], d.bext[1], d.bext[2]};
+    g.crust = d.crust;
+    g.circR = d.circR;
+  }
+  return g;
+}
+
+__device__ __forceinline__ ConvexGeo convex_of_obstacle(const BoxObstacle& B) {
+  ConvexGeo g;
+  g.type = kShapeBox;
+  g.nv = 0;
+  g.verts = nullptr;
+  g.r = 0.;
+  g.e = d3{B.extent[0], B.extent[1], B.extent[2]};
+  for (int j = 0; j < 9; ++j) g.B[j] = B.B[j];
+  g.o = d3{B.centre[0], B.centre[1], B.centre[2]};
+  const double* e = B.extent;
+  g.scaling = d3{e[0] < kEps ? 1. : (e[0] - B.crust) / e[0], e[1] < kEps ? 1. : (e[1] - B.crust) / e[1],
+                 e[2] < kEps ? 1. : (e[2] - B.crust) / e[2]};
+  g.bext = g.e;
+  g.crust = B.crust;
+  g.circR = norm(g.e);
+  return g;
+}
+
+template <class V, bool SHAPES = false>
 __device__ __forceinline__ void for_each_contact(const Particles& P, const Sorted& S, const Grid& G,
                                                  const BoxObstacle* __restrict__ obs, int nobs, DevStatus* st,
                                                  int start_base, int slot_lo, int slot_hi, int k, const Self& me,
-                                                 V& vis) {
+                                                 V& vis, const ConvexGeo* mg = nullptr, int cx = 0) {
   // own cell, clamped so that a particle flagged outside the grid cannot
   // index outside the cell table
   int ci = min(max(ref_cell(me.x.x, G, 0) - G.off[0], 0), G.n[0] - 1);
@@ -590,14 +704,28 @@ __device__ __forceinline__ void for_each_contact(const Particles& P, const Sorte
           const double co = P.crust[s2i];
           d3 Pc, ov;
           double d;
-          int res = sphere_sphere(me.x, me.r, me.c, AB, nab, ro, co, Pc, ov, d);
-          if (res == 0) continue;
-          if (res < 0) {
-            if (!recover_ss(me.x, me.r, me.c, xo, ro, co, Pc, ov, d)) {
-              raise_error(st, kContactError, me.id, S.sid[k2]);
-              continue;
+          if (SHAPES && (mg->type != kShapeSphere ||
+                         (P.shape[s2i] >= 0 && P.shapes[P.shape[s2i]].type != kShapeSphere))) {
+            ConvexGeo og = convex_of(P, s2i, xo, cx);
+            int res = closest_convex(*mg, og, Pc, ov, d);
+            if (res == 0) continue;
+            if (res < 0) {
+              if (!recover_gjk(*mg, og, Pc, ov, d)) {
+                raise_error(st, kContactError, me.id, S.sid[k2]);
+                continue;
+              }
+              atomicAdd(&st->n_recovered, 1);
             }
-            atomicAdd(&st->n_recovered, 1);
+          } else {
+            int res = sphere_sphere(me.x, me.r, me.c, AB, nab, ro, co, Pc, ov, d);
+            if (res == 0) continue;
+            if (res < 0) {
+              if (!recover_ss(me.x, me.r, me.c, xo, ro, co, Pc, ov, d)) {
+                raise_error(st, kContactError, me.id, S.sid[k2]);
+                continue;
+              }
+              atomicAdd(&st->n_recovered, 1);
+            }
           }
           if (!(d < 0.)) continue;
           vis.pair(k2, s2i, xo, Pc, ov, d);
@@ -615,14 +743,27 @@ __device__ __forceinline__ void for_each_contact(const Particles& P, const Sorte
     if (!probable) continue;
     d3 Pc, ov;
     double d;
-    int res = sphere_box(B, me.x, me.r, me.c, Pc, ov, d);
-    if (res == 0) continue;
-    if (res < 0) {
-      if (!recover_sb(B, me.x, me.r, me.c, Pc, ov, d)) {
-        raise_error(st, kContactError, me.id, B.id);
-        continue;
+    if (SHAPES && mg->type != kShapeSphere) {
+      ConvexGeo og = convex_of_obstacle(B);
+      int res = closest_convex(*mg, og, Pc, ov, d);
+      if (res == 0) continue;
+      if (res < 0) {
+        if (!recover_gjk(*mg, og, Pc, ov, d)) {
+          raise_error(st, kContactError, me.id, B.id);
+          continue;
+        }
+        atomicAdd(&st->n_recovered, 1);
       }
-      atomicAdd(&st->n_recovered, 1);
+    } else {
+      int res = sphere_box(B, me.x, me.r, me.c, Pc, ov, d);
+      if (res == 0) continue;
+      if (res < 0) {
+        if (!recover_sb(B, me.x, me.r, me.c, Pc, ov, d)) {
+          raise_error(st, kContactError, me.id, B.id);
+          continue;
+        }
+        atomicAdd(&st->n_recovered, 1);
+      }
     }
     if (!(d < 0.)) continue;
     vis.wall(o, Pc, ov, d);
@@ -696,14 +837,16 @@ struct ForceVisitor {
   }
 };
 
-template <bool MIXED>
+template <bool MIXED, bool SHAPES = false>
 __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted& S, const Grid& G,
                                                const BoxObstacle* __restrict__ obs, int nobs,
                                                const HookeModel* __restrict__ models, const StepParams& sp,
                                                DevStatus* st, int* pairs, int start_base, int slot_lo, int slot_hi,
-                                               int cv, int k, const Self& me, d3& F, d3& T) {
+                                               int cv, int k, const Self& me, d3& F, d3& T,
+                                               const ConvexGeo* mg = nullptr, int cx = 0) {
   ForceVisitor<MIXED> vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
-  for_each_contact(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis);
+  for_each_contact<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis,
+                                                mg, cx);
   F = vis.F;
   T = vis.T;
   if (MIXED) {

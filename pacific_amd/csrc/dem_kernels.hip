@@ -37,7 +37,16 @@ __global__ __launch_bounds__(256) void k_move(Particles P, Sorted S, Grid G, Ste
   d3 F = ld3(P.F, s), T = ld3(P.T, s);
   double mass = P.mass[s], cf = P.cfac[s], I = P.inertia[s];
   double depl;
-  if (sp.scheme == kLeapFrog) {
+  if (P.shape) {
+    // system with non-spherical particles (LeapFrog only): basis in place
+    double B[9];
+    for (int j = 0; j < 9; ++j) B[j] = P.B[cx][j][s];
+    const int sh = P.shape[s];
+    const bool gyro = sh >= 0 && P.shapes[sh].type != kShapeSphere;
+    const d3 I3 = gyro ? d3{P.I3[0][s], P.I3[1][s], P.I3[2][s]} : d3{I, I, I};
+    depl = lf_move_body(k, gyro, F, T, mass, cf, I, I3, dtv, dtx, B);
+    for (int j = 0; j < 9; ++j) P.B[cx][j][s] = B[j];
+  } else if (sp.scheme == kLeapFrog) {
     depl = lf_move(k, F, T, mass, cf, I, dtv, dtx);
   } else {
     AB2State ab;
@@ -178,7 +187,7 @@ __global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells
 // (the other schemes).
 enum { kForcesOnly = 0, kForcesKick = 1, kFused = 2 };
 
-template <int MODE, bool MIXED>
+template <int MODE, bool MIXED, bool SHAPES>
 __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
                                                 const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
                                                 int* pairs) {
@@ -203,7 +212,10 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   d3 F{sp.g[0] * wf, sp.g[1] * wf, sp.g[2] * wf};
   d3 T{0., 0., 0.};
 
-  sweep_particle<MIXED>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T);
+  ConvexGeo mg;
+  if (SHAPES) mg = convex_of(P, s, me.x, cx);
+  sweep_particle<MIXED, SHAPES>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
+                                cx);
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
   if (!(MODE == kFused && sp.scheme == kLeapFrog)) {
@@ -220,6 +232,26 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   kin.q = quat{d3{P.q[0][s], P.q[1][s], P.q[2][s]}, P.q[3][s]};
   const double cf = P.cfac[s], I = P.inertia[s];
   const int nv = cv ^ 1;
+  if (SHAPES) {
+    const bool gyro = mg.type != kShapeSphere;
+    const d3 I3 = gyro ? d3{P.I3[0][s], P.I3[1][s], P.I3[2][s]} : d3{I, I, I};
+    lf_kick_body(kin, gyro, F, T, me.m, cf, I, I3, 0.5 * sp.dt);
+    if (MODE == kFused) {
+      double depl = lf_move_body(kin, gyro, F, T, me.m, cf, I, I3, 0.5 * sp.dt, sp.dt, mg.B);
+      if (depl > me.c) raise_error(st, kMotionError, me.id, 0);
+      wrap_and_bin(kin, G, S, st, s, me.id);
+      st3(P.x[cx ^ 1], s, kin.x);
+      for (int j = 0; j < 9; ++j) P.B[cx ^ 1][j][s] = mg.B[j];
+      P.q[0][s] = kin.q.v.x;
+      P.q[1][s] = kin.q.v.y;
+      P.q[2][s] = kin.q.v.z;
+      P.q[3][s] = kin.q.w;
+    }
+    st3(P.v[nv], s, kin.v);
+    st3(P.w[nv], s, kin.w);
+    st3(P.wb, s, kin.wb);
+    return;
+  }
   if (MODE == kForcesKick) {
     lf_kick(kin, F, T, me.m, cf, I, 0.5 * sp.dt);
     st3(P.v[nv], s, kin.v);
@@ -305,8 +337,12 @@ template <int MODE, bool MIXED>
 static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
                             const HookeModel* models, const StepParams& sp, DevStatus* st, int* pairs,
                             hipStream_t stream) {
-  hipLaunchKernelGGL((k_forces<MODE, MIXED>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs, models,
-                     sp, st, pairs);
+  if (P.shape)
+    hipLaunchKernelGGL((k_forces<MODE, MIXED, true>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
+                       models, sp, st, pairs);
+  else
+    hipLaunchKernelGGL((k_forces<MODE, MIXED, false>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
+                       models, sp, st, pairs);
 }
 
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,

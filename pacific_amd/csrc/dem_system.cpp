@@ -12,6 +12,7 @@
 #include "../../include/grains_gpu.h"
 #include "dem_launch.hpp"
 #include "dem_math.hpp"
+#include "dem_shapes.hpp"
 #include "dem_system.hpp"
 
 using namespace dem;
@@ -108,6 +109,16 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, track(this, &S.sid, n));
   HIPCHECK(this, track(this, &S.posr, 4 * n));
 
+  if (has_shapes()) {
+    HIPCHECK(this, track(this, &P.shape, n));
+    for (int b = 0; b < 2; ++b)
+      for (int j = 0; j < 9; ++j) HIPCHECK(this, track(this, &P.B[b][j], n));
+    for (int a = 0; a < 3; ++a) HIPCHECK(this, track(this, &P.I3[a], n));
+    HIPCHECK(this, track(this, &d_shapes, shapes.size()));
+    HIPCHECK(this, track(this, &d_verts, shape_verts.size()));
+    P.shapes = d_shapes;
+    P.verts = d_verts;
+  }
   HIPCHECK(this, track(this, &d_obs, obstacles.size()));
   HIPCHECK(this, track(this, &d_models, (size_t)kMaxMaterials * kMaxMaterials));
   HIPCHECK(this, track(this, &d_status, 1));
@@ -202,6 +213,18 @@ int grains_gpu_system::upload() {
     HIPCHECK(this, hipMemcpy(d_obs, obstacles.data(), sizeof(BoxObstacle) * obstacles.size(),
                              hipMemcpyHostToDevice));
   HIPCHECK(this, hipMemcpy(d_models, models, sizeof(models), hipMemcpyHostToDevice));
+  if (has_shapes()) {
+    HIPCHECK(this, hipMemcpy(P.shape, h_shape.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    for (int j = 0; j < 9; ++j) {
+      HIPCHECK(this, up(P.B[0][j], h_B, 9, j));
+      HIPCHECK(this, up(P.B[1][j], h_B, 9, j));
+    }
+    for (int a = 0; a < 3; ++a) HIPCHECK(this, up(P.I3[a], h_I3, 3, a));
+    HIPCHECK(this, hipMemcpy(d_shapes, shapes.data(), sizeof(ShapeDesc) * shapes.size(), hipMemcpyHostToDevice));
+    if (!shape_verts.empty())
+      HIPCHECK(this, hipMemcpy(d_verts, shape_verts.data(), sizeof(double) * shape_verts.size(),
+                               hipMemcpyHostToDevice));
+  }
   cur = 0;
   return GRAINS_OK;
 }
@@ -344,6 +367,8 @@ int grains_gpu_system::initialize() {
           return fail(this, GRAINS_CONFIG_ERROR, buf);
         }
       }
+  if (has_shapes() && cfg.scheme != GRAINS_LEAPFROG)
+    return fail(this, GRAINS_CONFIG_ERROR, "non-spherical particles are integrated with SecondOrderLeapFrog only");
   int e = build_grid();
   if (e) return e;
   HIPCHECK(this, hipSetDevice(device));
@@ -442,8 +467,94 @@ int grains_gpu_add_spheres(grains_gpu_system* s, size_t n, const double* radius,
     s->h_q.push_back(0.);
     s->h_q.push_back(0.);
     s->h_q.push_back(1.);
+    s->h_shape.push_back(-1);
+    for (int j = 0; j < 9; ++j) s->h_B.push_back(j % 4 == 0 ? 1. : 0.);
+    for (int a = 0; a < 3; ++a) s->h_I3.push_back(I);
   }
   s->nparticles = (int)s->h_rad.size();
+  return GRAINS_OK;
+}
+
+int grains_gpu_add_shape(grains_gpu_system* s, int type, double radius, const double* L, int nv,
+                         const double* verts, int nf, const int* face_sizes, const int* face_idx, double crust,
+                         int* shape_id) {
+  if (!s || !shape_id) return GRAINS_ARGUMENT_ERROR;
+  if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "shapes must be added before initialisation");
+  ShapeBuild b;
+  std::string err;
+  if (build_shape(type, radius, L, nv, verts, nf, face_sizes, face_idx, crust, b, err))
+    return fail(s, GRAINS_CONFIG_ERROR, err);
+  b.desc.vert_off = (int)(s->shape_verts.size() / 3);
+  s->shape_verts.insert(s->shape_verts.end(), b.verts.begin(), b.verts.end());
+  s->shapes.push_back(b.desc);
+  s->shape_volume.push_back(b.volume);
+  for (int i = 0; i < 6; ++i) s->shape_inertia.push_back(b.inertia[i]);
+  *shape_id = (int)s->shapes.size() - 1;
+  return GRAINS_OK;
+}
+
+int grains_gpu_add_particles(grains_gpu_system* s, int shape, size_t n, const double* density, const int* material,
+                             const double* x, const double* v, const double* w, const double* basis) {
+  if (!s || !density || !material || !x) return GRAINS_ARGUMENT_ERROR;
+  if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "particles must be added before initialisation");
+  if (shape < 0 || shape >= (int)s->shapes.size()) return fail(s, GRAINS_ARGUMENT_ERROR, "unknown shape");
+  const ShapeDesc& d = s->shapes[shape];
+  const double vol = s->shape_volume[shape];
+  const double* Iu = &s->shape_inertia[6 * (size_t)shape];
+  for (size_t i = 0; i < n; ++i) {
+    const double rho = density[i];
+    if (!(rho > 0.)) return fail(s, GRAINS_CONFIG_ERROR, "density must be positive");
+    if (material[i] < 0 || material[i] >= kMaxMaterials) return fail(s, GRAINS_ARGUMENT_ERROR, "bad material");
+    // Particle(DOMNode*) (Particle.cpp:157-166): mass = rho V, inertia = rho I
+    const double mass = rho * vol;
+    double I[6];
+    for (int k = 0; k < 6; ++k) I[k] = Iu[k] * rho;
+    const double buoy = 1. - s->cfg.fluid_density / rho;
+    s->h_rad.push_back(d.type == kShapeSphere ? d.radius : d.circR);
+    s->h_mass.push_back(mass);
+    s->h_inertia.push_back(I[0]);
+    s->h_crust.push_back(d.crust);
+    s->h_wfac.push_back(mass * buoy);
+    double cf = 1.;
+    if (s->cfg.fluid_density != 0.) cf -= s->cfg.fluid_density / rho;
+    s->h_cfac.push_back(cf);
+    s->h_req.push_back(std::pow((0.75 / kPi) * vol, 1. / 3.));
+    s->h_mat.push_back(material[i]);
+    double Bm[9], q[4];
+    for (int j = 0; j < 9; ++j) Bm[j] = basis ? basis[9 * i + j] : (j % 4 == 0 ? 1. : 0.);
+    if (basis) quat_from_basis(Bm, q);
+    else q[0] = q[1] = q[2] = 0., q[3] = 1.;
+    quat qq{d3{q[0], q[1], q[2]}, q[3]};
+    d3 wv{0., 0., 0.};
+    for (int a = 0; a < 3; ++a) {
+      s->h_x.push_back(x[3 * i + a]);
+      s->h_v.push_back(v ? v[3 * i + a] : 0.);
+      s->h_w.push_back(w ? w[3 * i + a] : 0.);
+    }
+    if (w) wv = d3{w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+    d3 wb = to_body(qq, wv);
+    s->h_wb.push_back(wb.x);
+    s->h_wb.push_back(wb.y);
+    s->h_wb.push_back(wb.z);
+    for (int a = 0; a < 4; ++a) s->h_q.push_back(q[a]);
+    s->h_shape.push_back(shape);
+    for (int j = 0; j < 9; ++j) s->h_B.push_back(Bm[j]);
+    s->h_I3.push_back(I[0]);
+    s->h_I3.push_back(I[3]);
+    s->h_I3.push_back(I[5]);
+  }
+  s->nparticles = (int)s->h_rad.size();
+  return GRAINS_OK;
+}
+
+int grains_gpu_shape_info(grains_gpu_system* s, int shape, double* out) {
+  if (!s || !out || shape < 0 || shape >= (int)s->shapes.size()) return GRAINS_ARGUMENT_ERROR;
+  const ShapeDesc& d = s->shapes[shape];
+  out[0] = s->shape_volume[shape];
+  out[1] = d.circR;
+  for (int i = 0; i < 6; ++i) out[2 + i] = s->shape_inertia[6 * (size_t)shape + i];
+  for (int a = 0; a < 3; ++a) out[8 + a] = d.scaling[a];
+  for (int a = 0; a < 3; ++a) out[11 + a] = d.bext[a];
   return GRAINS_OK;
 }
 
@@ -603,6 +714,40 @@ int grains_gpu_get_state(grains_gpu_system* s, double* x, double* v, double* w, 
   }
   for (int a = 0; a < 4; ++a) HIPCHECK(s, down(q, s->P.q[a], 4, a));
   return s->check_status();
+}
+
+int grains_gpu_get_basis(grains_gpu_system* s, double* B) {
+  if (!s || !s->initialized || !B) return GRAINS_ARGUMENT_ERROR;
+  if (!s->has_shapes()) return fail(s, GRAINS_ARGUMENT_ERROR, "the system has no non-spherical particles");
+  HIPCHECK(s, hipSetDevice(s->device));
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  const int n = s->nparticles;
+  std::vector<int> ids((size_t)n);
+  HIPCHECK(s, hipMemcpy(ids.data(), s->P.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<double> buf((size_t)n);
+  const int cx = s->cur & 1;
+  for (int j = 0; j < 9; ++j) {
+    HIPCHECK(s, hipMemcpy(buf.data(), s->P.B[cx][j], sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) B[9 * (size_t)(ids[i] - 1) + j] = buf[i];
+  }
+  return s->check_status();
+}
+
+int grains_gpu_set_basis(grains_gpu_system* s, const double* B) {
+  if (!s || !s->initialized || !B) return GRAINS_ARGUMENT_ERROR;
+  if (!s->has_shapes()) return fail(s, GRAINS_ARGUMENT_ERROR, "the system has no non-spherical particles");
+  HIPCHECK(s, hipSetDevice(s->device));
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  const int n = s->nparticles;
+  std::vector<int> ids((size_t)n);
+  HIPCHECK(s, hipMemcpy(ids.data(), s->P.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<double> buf((size_t)n);
+  const int cx = s->cur & 1;
+  for (int j = 0; j < 9; ++j) {
+    for (int i = 0; i < n; ++i) buf[i] = B[9 * (size_t)(ids[i] - 1) + j];
+    HIPCHECK(s, hipMemcpy(s->P.B[cx][j], buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  return GRAINS_OK;
 }
 
 int grains_gpu_set_state(grains_gpu_system* s, const double* x, const double* v, const double* w, const double* q,

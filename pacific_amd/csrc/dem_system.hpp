@@ -31,6 +31,15 @@ struct grains_gpu_system {
   std::vector<double> h_rad, h_mass, h_inertia, h_crust, h_wfac, h_cfac, h_req;
   std::vector<double> h_x, h_v, h_w, h_wb, h_q;
   std::vector<int> h_mat;
+  // non-spherical particles: shape classes, their vertex table, and per
+  // particle the class (-1 = sphere added by add_spheres), basis rows and
+  // principal moments of inertia
+  std::vector<dem::ShapeDesc> shapes;
+  std::vector<double> shape_volume, shape_inertia;  // per unit density (1 / 6 per class)
+  std::vector<double> shape_verts;
+  std::vector<int> h_shape;
+  std::vector<double> h_B, h_I3;
+  bool has_shapes() const { return !shapes.empty(); }
   std::vector<dem::BoxObstacle> obstacles;
   dem::HookeModel models[dem::kMaxMaterials * dem::kMaxMaterials]{};
   dem::HookeModel default_model{};
@@ -41,6 +50,8 @@ struct grains_gpu_system {
   dem::Particles P{};
   dem::Sorted S{};
   dem::BoxObstacle* d_obs = nullptr;
+  dem::ShapeDesc* d_shapes = nullptr;
+  double* d_verts = nullptr;
   dem::HookeModel* d_models = nullptr;
   dem::DevStatus* d_status = nullptr;
   int* d_pairs = nullptr;

