@@ -219,6 +219,29 @@ int grains_gpu_batch_get_state(grains_gpu_batch* b, int isys, double* x, double*
                                double* T);
 const char* grains_gpu_batch_last_error(const grains_gpu_batch* b);
 
+/* ---- Coupling with a fluid solver ----------------------------------------
+ * Handle-based form of the Basilisk interface (InterfaceGrains3DBasilisk.h),
+ * returning status codes instead of ending the process. */
+
+/* Builds a system from a coupled input file: <TimeStep mindt maxdt ndt>,
+ * fluid density fluid_rho, acceleration corrected by (1 - rho_f / rho)
+ * unless fluid_corrected == 0; the Transform basis of every particle is kept
+ * (GrainsBuilderFactory::createCoupledWithFluid + do_before_time_stepping,
+ * InterfaceGrains3DBasilisk.cpp:18-45). */
+int grains_gpu_create_coupled(const char* xml_path, double fluid_rho, int fluid_corrected, grains_gpu_system** out);
+/* One fluid time step: max(min(maxdt, dt_fluid / ndt), mindt) sub-steps
+ * (GrainsCoupledWithFluid::Simulation, GrainsCoupledWithFluid.cpp:123-232);
+ * *nsub receives their number. */
+int grains_gpu_fluid_step(grains_gpu_system* sys, double dt_fluid, int* nsub);
+/* GrainsToFluid text (GrainsCoupledWithFluid.cpp:912-1120), tokens joined
+ * by single spaces; *buf is allocated with malloc (release with free). */
+int grains_gpu_to_fluid(grains_gpu_system* sys, char** buf, int* size);
+/* m rows of (vx, vy, vz, wx, wy, wz) by id - 1
+ * (GrainsCoupledWithFluid::updateParticlesVelocity, :1126-1181) */
+int grains_gpu_set_velocities(grains_gpu_system* sys, const double* vw, size_t m);
+/* particles and obstacles sent to the fluid (numberOfRBToFluid, :896-905) */
+int grains_gpu_count_rigid_bodies(grains_gpu_system* sys, size_t* nparticles, size_t* nobstacles);
+
 /* ---- Discrete adjoint ----------------------------------------------------
  * Reverse-mode derivative of nsteps SecondOrderLeapFrog steps (spheres, Hooke
  * model without rolling resistance, FP64) with respect to the initial

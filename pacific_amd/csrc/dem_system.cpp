@@ -448,7 +448,9 @@ int grains_gpu_add_spheres(grains_gpu_system* s, size_t n, const double* radius,
     s->h_wfac.push_back(mass * buoy);  // Particle::computeWeight (Particle.cpp:413-418)
     double cf = 1.;
     if (s->cfg.fluid_density != 0.) cf -= s->cfg.fluid_density / rho;  // ParticleKinematics.cpp:485-487
+    if (!s->fluid_corrected) cf = 1.;  // Particle::setFluidCorrectedAcceleration(false)
     s->h_cfac.push_back(cf);
+    s->h_density.push_back(rho);
     s->h_req.push_back(std::pow((0.75 / kPi) * volume, 1. / 3.));  // Component.cpp:205-208
     s->h_mat.push_back(material[i]);
     d3 wv{0., 0., 0.};
@@ -486,6 +488,16 @@ int grains_gpu_add_shape(grains_gpu_system* s, int type, double radius, const do
     return fail(s, GRAINS_CONFIG_ERROR, err);
   b.desc.vert_off = (int)(s->shape_verts.size() / 3);
   s->shape_verts.insert(s->shape_verts.end(), b.verts.begin(), b.verts.end());
+  std::vector<int> faces;
+  if (type == kShapePoly) {
+    int off = 0;
+    for (int f = 0; f < nf; ++f) {
+      faces.push_back(face_sizes[f]);
+      for (int j = 0; j < face_sizes[f]; ++j) faces.push_back(face_idx[off + j]);
+      off += face_sizes[f];
+    }
+  }
+  s->shape_faces.push_back(faces);
   s->shapes.push_back(b.desc);
   s->shape_volume.push_back(b.volume);
   for (int i = 0; i < 6; ++i) s->shape_inertia.push_back(b.inertia[i]);
@@ -517,7 +529,9 @@ int grains_gpu_add_particles(grains_gpu_system* s, int shape, size_t n, const do
     s->h_wfac.push_back(mass * buoy);
     double cf = 1.;
     if (s->cfg.fluid_density != 0.) cf -= s->cfg.fluid_density / rho;
+    if (!s->fluid_corrected) cf = 1.;
     s->h_cfac.push_back(cf);
+    s->h_density.push_back(rho);
     s->h_req.push_back(std::pow((0.75 / kPi) * vol, 1. / 3.));
     s->h_mat.push_back(material[i]);
     double Bm[9], q[4];
@@ -602,6 +616,7 @@ int grains_gpu_add_box(grains_gpu_system* s, const double* L, const double* cent
     o.aabb_c[a] = mn[a] + o.aabb_e[a];
   }
   s->obstacles.push_back(o);
+  s->obs_to_fluid.push_back(0);
   return GRAINS_OK;
 }
 

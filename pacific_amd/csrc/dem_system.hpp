@@ -39,7 +39,18 @@ struct grains_gpu_system {
   std::vector<double> shape_verts;
   std::vector<int> h_shape;
   std::vector<double> h_B, h_I3;
-  bool has_shapes() const { return !shapes.empty(); }
+  std::vector<double> h_density;
+  std::vector<std::vector<int>> shape_faces;  // per class: face vertex lists (polyhedra)
+  // the Transform basis is also tracked for spheres (coupled runs report it)
+  bool track_basis = false;
+  bool has_shapes() const { return !shapes.empty() || track_basis; }
+  // coupling with a fluid solver (GrainsCoupledWithFluid): the acceleration
+  // is corrected by (1 - rho_f / rho) unless switched off, and the DEM time
+  // step follows <TimeStep mindt maxdt ndt>
+  bool fluid_corrected = true;
+  double coupled_min_dt = 0., coupled_max_dt = 0.;
+  size_t coupled_ndt = 0;
+  std::vector<int> obs_to_fluid;
   std::vector<dem::BoxObstacle> obstacles;
   dem::HookeModel models[dem::kMaxMaterials * dem::kMaxMaterials]{};
   dem::HookeModel default_model{};

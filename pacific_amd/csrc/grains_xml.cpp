@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/grains_gpu.h"
+#include "grains_xml.hpp"
 #include "dem_system.hpp"
 
 namespace {
@@ -339,6 +340,10 @@ bool read_polyhedron(const std::string& path, ParticleClass& pc, std::string& er
 }  // namespace
 
 extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, grains_gpu_system** out) {
+  return dem::create_from_xml(xml_path, precision, dem::XmlOptions{}, out);
+}
+
+int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& opt, grains_gpu_system** out) {
   if (!xml_path || !out) return GRAINS_ARGUMENT_ERROR;
   *out = nullptr;
   std::ifstream f(xml_path);
@@ -374,7 +379,7 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
   std::vector<ParticleClass> classes;
   struct BoxDef {
     double L[3], centre[3], B[9], crust;
-    int mat;
+    int mat, to_fluid;
   };
   std::vector<BoxDef> boxes;
   struct ModelDef {
@@ -388,10 +393,13 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
   double amp_t = 0., amp_r = 0.;
   unsigned seed = 0;
   double t_end = 0.;
+  double coupled_dt[2] = {0., 0.};
+  size_t coupled_ndt = 0;
 
   if (c.ok && root.name != "Grains3D") c.bad("root element must be <Grains3D>");
-  if (c.ok && root.has("Type") && root.attr("Type") != "Standard")
+  if (c.ok && !opt.coupled && root.has("Type") && root.attr("Type") != "Standard")
     c.bad("only <Grains3D Type=\"Standard\"> is supported");
+  cfg.fluid_density = opt.fluid_density;
   const XNode* cons = c.ok ? root.child("Construction") : nullptr;
   if (c.ok && !cons) c.bad("missing <Construction>");
 
@@ -482,6 +490,9 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
         basis_of(n, b.B, c);
         const XNode* m = n->child("Material");
         b.mat = mats.define(m ? trim(m->text) : std::string(""), c);
+        // <Status ToFluid="1"/> (SimpleObstacle.cpp:54-57)
+        const XNode* stn = n->child("Status");
+        b.to_fluid = stn ? std::atoi(stn->attr("ToFluid").c_str()) : 0;
         boxes.push_back(b);
       }
     }
@@ -533,9 +544,21 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
     if (!sim) c.bad("missing <Simulation>");
     else {
       const XNode* ti = sim->child("TimeInterval");
-      cfg.t_start = dattr(ti, "Start", c);
-      t_end = dattr(ti, "End", c);
-      cfg.dt = dattr(sim->child("TimeStep"), "dt", c);
+      if (opt.coupled) {
+        // GrainsCoupledWithFluid::Construction (GrainsCoupledWithFluid.cpp:
+        // 509-527): <TimeStep mindt maxdt ndt>, no time interval
+        const XNode* ts = sim->child("TimeStep");
+        coupled_dt[0] = dattr(ts, "mindt", c);
+        coupled_dt[1] = dattr(ts, "maxdt", c);
+        coupled_ndt = (size_t)std::atol(ts ? ts->attr("ndt").c_str() : "0");
+        if (c.ok && (!(coupled_dt[0] > 0.) || coupled_dt[1] < coupled_dt[0] || coupled_ndt == 0))
+          c.bad("<TimeStep> needs 0 < mindt <= maxdt and ndt >= 1");
+        cfg.dt = coupled_dt[0];
+      } else {
+        cfg.t_start = dattr(ti, "Start", c);
+        t_end = dattr(ti, "End", c);
+        cfg.dt = dattr(sim->child("TimeStep"), "dt", c);
+      }
       if (const XNode* tint = sim->child("TimeIntegration")) {
         std::string t = tint->attr("Type");
         if (t == "SecondOrderLeapFrog") cfg.scheme = GRAINS_LEAPFROG;
@@ -631,6 +654,11 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
     return GRAINS_CONFIG_ERROR;
   }
   s->t_end = t_end;
+  s->track_basis = opt.track_basis;
+  s->fluid_corrected = opt.fluid_corrected;
+  s->coupled_min_dt = coupled_dt[0];
+  s->coupled_max_dt = coupled_dt[1];
+  s->coupled_ndt = coupled_ndt;
   int e;
   for (auto& m : models) {
     e = grains_gpu_set_contact_model(s, m.a, m.b, m.kn, m.en, m.etat, m.muc, m.kr);
@@ -681,6 +709,7 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
   for (auto& b : boxes) {
     e = grains_gpu_add_box(s, b.L, b.centre, b.B, b.crust, b.mat);
     if (e) return e;
+    s->obs_to_fluid.back() = b.to_fluid;
   }
   return grains_gpu_initialize(s);
 }

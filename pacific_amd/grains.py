@@ -43,7 +43,8 @@ EXPORTS = [
     "grains_gpu_batch_time", "grains_gpu_batch_get_state", "grains_gpu_batch_last_error",
     "grains_gpu_adjoint_forward", "grains_gpu_adjoint_backward", "grains_contact_eval",
     "grains_gpu_contact_counts", "grains_gpu_add_shape", "grains_gpu_add_particles", "grains_gpu_shape_info",
-    "grains_gpu_get_basis", "grains_gpu_set_basis",
+    "grains_gpu_get_basis", "grains_gpu_set_basis", "grains_gpu_create_coupled", "grains_gpu_fluid_step",
+    "grains_gpu_to_fluid", "grains_gpu_set_velocities", "grains_gpu_count_rigid_bodies",
 ]
 
 
@@ -110,12 +111,23 @@ def lib():
         L.grains_gpu_shape_info.argtypes = [vp, ctypes.c_int, dp]
         L.grains_gpu_get_basis.argtypes = [vp, dp]
         L.grains_gpu_set_basis.argtypes = [vp, dp]
+        L.grains_gpu_create_coupled.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(vp)]
+        L.grains_gpu_fluid_step.argtypes = [vp, ctypes.c_double, ip]
+        L.grains_gpu_to_fluid.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), ip]
+        L.grains_gpu_set_velocities.argtypes = [vp, dp, sz]
+        L.grains_gpu_count_rigid_bodies.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
 
 def _d(a):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _libc_free(p):
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    libc.free(p)
 
 
 def _c64(a, shape=None):
@@ -173,6 +185,44 @@ class GrainsGPU:
         cfg.precision = precision
         cfg.device = device
         self._check(L.grains_gpu_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    @classmethod
+    def coupled(cls, path, fluid_rho, fluid_corrected=True):
+        """System of a fluid-coupled input file (grains_gpu_create_coupled)."""
+        L = lib()
+        h = ctypes.c_void_p()
+        code = L.grains_gpu_create_coupled(os.fsencode(path), float(fluid_rho), 1 if fluid_corrected else 0,
+                                           ctypes.byref(h))
+        if code:
+            msg = L.grains_gpu_last_error(h).decode() if h.value else "cannot build the system"
+            if h.value:
+                L.grains_gpu_destroy(h)
+            raise GrainsError(code, msg)
+        return cls(None, None, _handle=h)
+
+    def fluid_step(self, dt_fluid):
+        n = ctypes.c_int()
+        self._check(lib().grains_gpu_fluid_step(self.h, float(dt_fluid), ctypes.byref(n)))
+        return n.value
+
+    def to_fluid(self):
+        """GrainsToFluid text (tokens joined by single spaces)."""
+        L = lib()
+        buf, size = ctypes.c_void_p(), ctypes.c_int()
+        self._check(L.grains_gpu_to_fluid(self.h, ctypes.byref(buf), ctypes.byref(size)))
+        try:
+            return ctypes.string_at(buf.value).decode()
+        finally:
+            _libc_free(buf)
+
+    def set_velocities(self, vw):
+        vw = _c64(vw, (-1, 6))
+        self._check(lib().grains_gpu_set_velocities(self.h, _d(vw), len(vw)))
+
+    def count_rigid_bodies(self):
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        self._check(lib().grains_gpu_count_rigid_bodies(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     @classmethod
     def from_xml(cls, path, precision=FP64):
