@@ -117,6 +117,14 @@ int grains_gpu_add_particles(grains_gpu_system* sys, int shape, size_t n, const 
 /* out = {volume, circumscribed radius, inertia xx xy xz yy yz zz, crust
  * scaling[3], bounding-box half extents[3]}, per unit density */
 int grains_gpu_shape_info(grains_gpu_system* sys, int shape, double* out);
+/* The device narrow phase evaluated on the host for two bodies of shape
+ * classes shapeA, shapeB at centres x and row-major bases B
+ * (RigidBodyWithCrust::ClosestPoint and its ContactError handling,
+ * RigidBodyWithCrust.cpp:205-354, 534-590): out = {contact point[3],
+ * overlap vector[3], overlap distance, status} with status 1 features set,
+ * 0 no contact, -1 ContactError, 2 recovered.  For checks without a device. */
+int grains_gpu_closest_point(grains_gpu_system* sys, int shapeA, const double* xA, const double* BA, int shapeB,
+                             const double* xB, const double* BB, double* out);
 /* Transform bases of all particles (id order, 9 per particle, row-major) */
 int grains_gpu_get_basis(grains_gpu_system* sys, double* basis);
 int grains_gpu_set_basis(grains_gpu_system* sys, const double* basis);
@@ -173,6 +181,8 @@ int grains_gpu_set_record_pairs(grains_gpu_system* sys, int on);
 int grains_gpu_get_pairs(grains_gpu_system* sys, int* buf, size_t cap, size_t* n);
 
 size_t grains_gpu_num_particles(const grains_gpu_system* sys);
+/* <TimeInterval Start End> and <TimeStep dt> of the system */
+int grains_gpu_time_interval(const grains_gpu_system* sys, double* t_start, double* t_end, double* dt);
 double grains_gpu_time(const grains_gpu_system* sys);
 void grains_gpu_set_time(grains_gpu_system* sys, double t);
 /* Directed contact records and, of those, contacts on the Coulomb cap

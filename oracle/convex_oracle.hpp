@@ -9,10 +9,7 @@
 // Parity notes: the Johnson subalgorithm, the termination tests and the
 // witness-point formula follow Convex.cpp:219-627 step by step (same
 // operation order), so the distance and witness points are pinned wherever
-// the support points are unique.  Polyhedron::support hill-climbs from a warm
-// start kept per particle (Polyhedron.cpp:265-293); among tied maxima (faces
-// exactly parallel to the search direction) the vertex returned depends on
-// that warm start, which this restatement keeps per particle as well.
+// the support points are unique.  Polyhedron support: see supportPoly.
 
 enum ShapeType { SH_SPHERE = 0, SH_BOX = 1, SH_POLY = 2 };
 
@@ -47,22 +44,26 @@ inline V3 supportBox(V3 const& e, V3 const& v) {
   if (n < EPSILON) return V3();
   return V3(v[0] < 0. ? -e[0] : e[0], v[1] < 0. ? -e[1] : e[1], v[2] < 0. ? -e[2] : e[2]);
 }
-// Polyhedron::support (Polyhedron.cpp:265-293), hill climbing from `curr`
+// Polyhedron support as Polyhedron::support20 (Polyhedron.cpp:431-439): the
+// first vertex of maximal projection.  The reference hill-climbs from a
+// per-instance warm start instead (Polyhedron.cpp:265-293, moving only on
+// improvements above 1e-16); both return a maximising vertex, and differ
+// only among vertices tied to within 1e-16 -- which GJK meets whenever it
+// converges onto a face normal, e.g. a sphere resting on a face.  The first
+// maximum is what the device computes; the warm-start history of the
+// reference is path dependent and not reproduced (parity unpinned there).
 inline V3 supportPoly(Shape const& sh, int& curr, V3 const& v) {
+  (void)curr;
   double n = Norm(v);
   if (!(n > EPSILON)) return V3();
-  int last = -1;
-  double h = dot(sh.verts[curr], v), d = 0.;
-  for (;;) {
-    const std::vector<int>& cb = sh.cobound[curr];
-    size_t i = 0, m = cb.size();
-    while (i != m && (cb[i] == last || (d = dot(sh.verts[cb[i]], v)) - h < 1.e-16)) ++i;
-    if (i == m) break;
-    last = curr;
-    curr = cb[i];
-    h = d;
-  }
-  return sh.verts[curr];
+  int c = 0;
+  double h = dot(sh.verts[0], v), d;
+  for (size_t i = 1; i < sh.verts.size(); ++i)
+    if ((d = dot(sh.verts[i], v)) > h) {
+      c = (int)i;
+      h = d;
+    }
+  return sh.verts[c];
 }
 
 // A convex in a GJK call: its shape, radius for spheres and the warm start

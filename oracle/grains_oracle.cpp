@@ -1490,6 +1490,50 @@ double orc_gjk(void* h, int shapeA, const double* BA, const double* oA, int shap
   return d;
 }
 
+// The oracle's narrow phase for two bodies of shape classes placed at
+// centres x with bases B: out = {P[3], ov[3], d, status} with status 1
+// features set, 0 no contact, -1 ContactError, 2 recovered.  Spheres of a
+// sphere class go through the sphere-sphere closed form as particles added
+// with orc_add_sphere do.
+int orc_closest_point(void* h, int shapeA, const double* xA, const double* BA, int shapeB, const double* xB,
+                      const double* BB, double* out) {
+  System* s = (System*)h;
+  auto mk = [&](int sh, const double* x, const double* B) {
+    Shape const& d = s->shapes[sh];
+    Part p;
+    p.shape = d.type == SH_SPHERE ? -1 : sh;
+    p.radius = d.type == SH_SPHERE ? d.radius : d.circR;
+    p.circR = d.circR;
+    p.crust = d.crust;
+    p.curr = d.start_vertex;
+    p.x = V3(x[0], x[1], x[2]);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) p.B[i][j] = B[3 * i + j];
+    return p;
+  };
+  Part A = mk(shapeA, xA, BA), B = mk(shapeB, xB, BB);
+  Contact c;
+  c.any = false;
+  c.d = 1.e20;
+  int st;
+  if (A.shape < 0 && B.shape < 0) {
+    st = sphereSphere(A, A.x, B, B.x, c) ? 1 : -1;
+    if (st < 0 && recoverSphereSphere(A, A.x, B, B.x, c)) st = 2;
+  } else {
+    Geo ga = geoOf(s, A, A.x), gb = geoOf(s, B, B.x);
+    st = closestPointConvex(ga, gb, c) ? 1 : -1;
+    if (st < 0 && closestPointRecoverGJK(ga, gb, c)) st = 2;
+  }
+  if (st == 1 && !c.any) st = 0;
+  for (int a = 0; a < 3; ++a) {
+    out[a] = c.any ? c.P[a] : 0.;
+    out[3 + a] = c.any ? c.ov[a] : 0.;
+  }
+  out[6] = c.any ? c.d : 1.e20;
+  out[7] = st;
+  return 0;
+}
+
 // Particle bases (id order, 9 per particle, row-major)
 void orc_get_basis(void* h, double* B) {
   System* s = (System*)h;

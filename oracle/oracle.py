@@ -62,6 +62,7 @@ def lib():
         L.orc_gjk.restype = ctypes.c_double
         L.orc_get_basis.argtypes = [vp, dp]
         L.orc_set_basis.argtypes = [vp, dp]
+        L.orc_closest_point.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, dp]
         L.orc_gjk_stats.argtypes = [ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
         _lib = L
     return _lib
@@ -183,6 +184,13 @@ class Oracle:
         B = np.zeros((self.n, 9))
         lib().orc_get_basis(self.h, _ptr(B))
         return B.reshape(self.n, 3, 3)
+
+    def closest_point(self, shapeA, xA, BA, shapeB, xB, BB):
+        out = np.zeros(8)
+        lib().orc_closest_point(self.h, int(shapeA), _d(np.asarray(xA, np.float64)),
+                                _d(np.asarray(BA, np.float64).reshape(9)), int(shapeB),
+                                _d(np.asarray(xB, np.float64)), _d(np.asarray(BB, np.float64).reshape(9)), _ptr(out))
+        return out[0:3].copy(), out[3:6].copy(), out[6], int(out[7])
 
     def set_basis(self, B):
         lib().orc_set_basis(self.h, _d(np.asarray(B, np.float64).reshape(self.n, 9)))

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/grains_gpu.h"
+#include "dem_device.hpp"
 #include "dem_launch.hpp"
 #include "dem_math.hpp"
 #include "dem_shapes.hpp"
@@ -561,6 +562,47 @@ int grains_gpu_add_particles(grains_gpu_system* s, int shape, size_t n, const do
   return GRAINS_OK;
 }
 
+int grains_gpu_closest_point(grains_gpu_system* s, int shapeA, const double* xA, const double* BA, int shapeB,
+                             const double* xB, const double* BB, double* out) {
+  if (!s || !xA || !BA || !xB || !BB || !out) return GRAINS_ARGUMENT_ERROR;
+  const int ns = (int)s->shapes.size();
+  if (shapeA < 0 || shapeA >= ns || shapeB < 0 || shapeB >= ns) return fail(s, GRAINS_ARGUMENT_ERROR, "unknown shape");
+  auto geo = [&](int sh, const double* x, const double* B) {
+    const ShapeDesc& d = s->shapes[sh];
+    ConvexGeo g;
+    g.type = d.type;
+    g.nv = d.nv;
+    g.verts = s->shape_verts.data() + 3 * (size_t)d.vert_off;
+    g.r = d.radius;
+    g.e = d3{d.extent[0], d.extent[1], d.extent[2]};
+    for (int j = 0; j < 9; ++j) g.B[j] = B[j];
+    g.o = d3{x[0], x[1], x[2]};
+    g.scaling = d3{d.scaling[0], d.scaling[1], d.scaling[2]};
+    g.bext = d3{d.bext[0], d.bext[1], d.bext[2]};
+    g.crust = d.crust;
+    g.circR = d.circR;
+    return g;
+  };
+  ConvexGeo A = geo(shapeA, xA, BA), B = geo(shapeB, xB, BB);
+  d3 P{0., 0., 0.}, ov{0., 0., 0.};
+  double dd = 1e20;
+  int res;
+  if (A.type == kShapeSphere && B.type == kShapeSphere) {
+    d3 AB = B.o - A.o;
+    double nab = norm(AB);
+    res = sphere_sphere(A.o, A.r, A.crust, AB, nab, B.r, B.crust, P, ov, dd);
+    if (res < 0 && recover_ss(A.o, A.r, A.crust, B.o, B.r, B.crust, P, ov, dd)) res = 2;
+  } else {
+    res = closest_convex(A, B, P, ov, dd);
+    if (res < 0 && recover_gjk(A, B, P, ov, dd)) res = 2;
+  }
+  out[0] = P.x; out[1] = P.y; out[2] = P.z;
+  out[3] = ov.x; out[4] = ov.y; out[5] = ov.z;
+  out[6] = dd;
+  out[7] = res;
+  return GRAINS_OK;
+}
+
 int grains_gpu_shape_info(grains_gpu_system* s, int shape, double* out) {
   if (!s || !out || shape < 0 || shape >= (int)s->shapes.size()) return GRAINS_ARGUMENT_ERROR;
   const ShapeDesc& d = s->shapes[shape];
@@ -856,6 +898,14 @@ int grains_gpu_contact_counts(grains_gpu_system* s, size_t* contacts, size_t* sl
   if (e) return e;
   if (contacts) *contacts = (size_t)s->last_status.npairs;
   if (sliding) *sliding = (size_t)s->last_status.n_slip;
+  return GRAINS_OK;
+}
+
+int grains_gpu_time_interval(const grains_gpu_system* s, double* t_start, double* t_end, double* dt) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  if (t_start) *t_start = s->cfg.t_start;
+  if (t_end) *t_end = s->t_end;
+  if (dt) *dt = s->cfg.dt;
   return GRAINS_OK;
 }
 

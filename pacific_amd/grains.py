@@ -45,6 +45,7 @@ EXPORTS = [
     "grains_gpu_contact_counts", "grains_gpu_add_shape", "grains_gpu_add_particles", "grains_gpu_shape_info",
     "grains_gpu_get_basis", "grains_gpu_set_basis", "grains_gpu_create_coupled", "grains_gpu_fluid_step",
     "grains_gpu_to_fluid", "grains_gpu_set_velocities", "grains_gpu_count_rigid_bodies",
+    "grains_gpu_time_interval", "grains_gpu_closest_point",
 ]
 
 
@@ -111,6 +112,7 @@ def lib():
         L.grains_gpu_shape_info.argtypes = [vp, ctypes.c_int, dp]
         L.grains_gpu_get_basis.argtypes = [vp, dp]
         L.grains_gpu_set_basis.argtypes = [vp, dp]
+        L.grains_gpu_closest_point.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, dp]
         L.grains_gpu_create_coupled.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(vp)]
         L.grains_gpu_fluid_step.argtypes = [vp, ctypes.c_double, ip]
         L.grains_gpu_to_fluid.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), ip]
@@ -310,6 +312,14 @@ class GrainsGPU:
         self._check(lib().grains_gpu_shape_info(self.h, int(shape), _d(out)))
         return {"volume": out[0], "circR": out[1], "inertia": out[2:8].copy(), "scaling": out[8:11].copy(),
                 "bbox_extent": out[11:14].copy()}
+
+    def closest_point(self, shapeA, xA, BA, shapeB, xB, BB):
+        """Host evaluation of the device narrow phase: (P, ov, d, status)."""
+        out = np.zeros(8)
+        args = [_c64(a).reshape(-1) for a in (xA, BA, xB, BB)]
+        self._check(lib().grains_gpu_closest_point(self.h, int(shapeA), _d(args[0]), _d(args[1]), int(shapeB),
+                                                   _d(args[2]), _d(args[3]), _d(out)))
+        return out[0:3].copy(), out[3:6].copy(), out[6], int(out[7])
 
     def basis(self):
         B = np.zeros((self.n, 9))

@@ -56,7 +56,7 @@ def tetrahedron(circumradius):
     v = np.array([[1.0, 1.0, 1.0], [1.0, -1.0, -1.0], [-1.0, 1.0, -1.0], [-1.0, -1.0, 1.0]])
     v *= circumradius / np.sqrt(3.0)
     faces = [[0, 1, 2], [0, 3, 1], [0, 2, 3], [1, 3, 2]]
-    return v, faces
+    return v, outward(v, faces)
 
 
 def icosahedron(circumradius):
@@ -68,7 +68,21 @@ def icosahedron(circumradius):
     faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
              [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
              [6, 2, 10], [8, 6, 7], [9, 8, 1]]
-    return v, faces
+    return v, outward(v, faces)
+
+
+def outward(verts, faces):
+    """faces re-ordered counter-clockwise seen from outside (the polyhedron
+    is centred on the origin), so that every edge is walked once in each
+    direction -- what the vertex adjacency of Polyhedron::BuildPolyhedron
+    (Polyhedron.cpp:324-359) needs to list every neighbour"""
+    out = []
+    for f in faces:
+        a, b, c = (np.asarray(verts[i]) for i in f[:3])
+        if np.dot(np.cross(b - a, c - a), a + b + c) < 0:
+            f = [f[0]] + list(reversed(f[1:]))
+        out.append(list(f))
+    return out
 
 
 def quat_to_matrix(q):
