@@ -39,11 +39,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+NBR_CAP = 32  # neighbour candidates per particle (0: the force sweep searches the 27 cells)
+
+
 def build_gpu(wl, precision):
     from pacific_amd import GrainsGPU, workloads
     g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], 0.0, wl["dt"],
                   0, precision)
     workloads.build(g, wl)
+    g.neighbor_list(NBR_CAP)
     g.init()
     return g
 
@@ -193,6 +197,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=400)
+    ap.add_argument("--nbr-cap", type=int, default=32,
+                    help="neighbour candidates per particle of the per-step list (0: 27-cell sweep in the force kernel)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost-ratio leg")
@@ -206,6 +212,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
+    global NBR_CAP
+    NBR_CAP = args.nbr_cap
 
     if args.calibrate_copy:
         # a plain copy of known size: the rocprofv3 FETCH_SIZE / WRITE_SIZE of
@@ -251,10 +259,11 @@ def main():
     prof = g.profile_steps(args.profile_steps)
     force_ms = prof["force_ms"] / max(prof["n_force"], 1)
     bin_ms = prof["bin_ms"] / max(prof["n_bin"], 1)
+    detect_ms = prof["detect_ms"] / max(prof["n_detect"], 1) if prof["n_detect"] else 0.0
     b_alg = (B_ALG[args.precision] + B_PAIR * cbar) * args.n
     achieved = b_alg / (force_ms / 1e3) / 1e9
-    log(f"bench: force sweep {force_ms * 1e3:.1f} us/launch, binning {bin_ms * 1e3:.1f} us/step, "
-        f"contacts/particle {cbar:.4f}")
+    log(f"bench: force sweep {force_ms * 1e3:.1f} us/launch, neighbour search {detect_ms * 1e3:.1f} us/launch, "
+        f"binning {bin_ms * 1e3:.1f} us/step, contacts/particle {cbar:.4f}")
 
     extra = {}
     if not args.no_fp64 and prec != FP64:
@@ -293,7 +302,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"k_forces<fused,{args.precision}>", "kernel_ms": force_ms,
-                     "bytes_alg_per_launch": b_alg, "contacts_per_particle": cbar, "binning_ms": bin_ms},
+                     "bytes_alg_per_launch": b_alg, "contacts_per_particle": cbar, "binning_ms": bin_ms,
+                     "neighbour_search_ms": detect_ms},
     }
     out.update(extra)
     if not args.no_cpu:

@@ -149,7 +149,8 @@ int grains_gpu_step_timed(grains_gpu_system* sys, int nsteps, double* ms);
 /* Same, with HIP events around every force sweep and every binning sequence
  * on the system's stream: out[0] total ms, out[1] summed force-sweep ms,
  * out[2] number of force sweeps, out[3] summed binning ms, out[4] number of
- * binning sequences.  No counterpart in the reference (instrumentation of
+ * binning sequences, out[5] summed neighbour-search ms, out[6] number of
+ * neighbour searches.  No counterpart in the reference (instrumentation of
  * the Grains::Simulation timers, solvercomputingtime.cpp:48-215). */
 int grains_gpu_profile_steps(grains_gpu_system* sys, int nsteps, double* out);
 
@@ -173,6 +174,13 @@ int grains_gpu_set_state(grains_gpu_system* sys, const double* x, const double* 
 /* mass, sphere moment of inertia and weight (3 per particle) as built from
  * radius and density (Particle.cpp:157-166, 413-418). */
 int grains_gpu_get_masses(grains_gpu_system* sys, double* mass, double* inertia, double* weight);
+
+/* Capacity of the per-step neighbour candidate list (default 32 per
+ * particle; 0 = none: the force sweep searches the 27 cells itself).  The
+ * broad phase of LinkedCell::ComputeForces (LinkedCell.cpp:1019-1103) then
+ * runs as its own launch; forces are bit-identical either way.  Before
+ * grains_gpu_initialize only. */
+int grains_gpu_set_neighbor_list(grains_gpu_system* sys, int capacity);
 
 /* Contact records of the last force evaluation: pairs (receiver id, partner
  * id), one per particle that received a contact force from the partner

@@ -564,21 +564,7 @@ __device__ __forceinline__ void record_pair(const StepParams& sp, DevStatus* st,
   }
 }
 
-// Enumerates the contacts (overlap < 0) of one particle (sorted slot k) with
-// its 27 neighbour cells and the obstacles -- LinkedCell::ComputeForces +
-// Particle::InterAction + SimpleObstacle::InterAction (LinkedCell.cpp:1019-1103,
-// Particle.cpp:477-545, SimpleObstacle.cpp:222-271) -- and hands each one to
-// the visitor:
-//   vis.pair(k2, s2i, xo, Pc, ov, d)   neighbour in sorted slot k2, storage
-//                                      s2i, seen at position xo (periodic image)
-//   vis.wall(o, Pc, ov, d)             obstacle o
-// with the contact point Pc, overlap vector ov and overlap distance d < 0 of
-// RigidBodyWithCrust::ClosestPoint (body A = this particle).  start_base
-// offsets the cell-start table; [slot_lo, slot_hi) bounds the sorted slots and
-// storage indices of the particle's system (one system: 0 and [0, N)).
-// With SHAPES the system holds non-spherical particles: mg describes this
-// particle's body (its shape, basis and crust) and cx selects the basis
-// buffer of the neighbours.
+// Body of particle s (storage index) at position x, basis buffer cx.
 __device__ __forceinline__ ConvexGeo convex_of(const Particles& P, int s, d3 x, int cx) {
   ConvexGeo g;
   const int sh = P.shape[s];
@@ -629,21 +615,26 @@ __device__ __forceinline__ ConvexGeo convex_of_obstacle(const BoxObstacle& B) {
   return g;
 }
 
-template <class V, bool SHAPES = false>
-__device__ __forceinline__ void for_each_contact(const Particles& P, const Sorted& S, const Grid& G,
-                                                 const BoxObstacle* __restrict__ obs, int nobs, DevStatus* st,
-                                                 int start_base, int slot_lo, int slot_hi, int k, const Self& me,
-                                                 V& vis, const ConvexGeo* mg = nullptr, int cx = 0) {
+// Periodic image offset of a neighbour seen across a face: -L, 0 or +L on
+// axis a for image code -1, 0, +1.
+__device__ __forceinline__ double image_shift(const Grid& G, int a, int c) {
+  return c < 0 ? -G.L[a] : (c > 0 ? G.L[a] : 0.);
+}
+
+// Enumerates the neighbour candidates of a particle at x: every sorted slot
+// k2 of its 27 neighbour cells, with the periodic image (ix, iy, iz) under
+// which it is seen, in a fixed order (rows (dj, dk) of x-adjacent cells,
+// each row's unshifted run, then its cells across the periodic x faces).
+// Cells of one row are consecutive keys, so each row is one contiguous run
+// of sorted slots; the nine row ranges are loaded before any candidate.
+template <class F>
+__device__ __forceinline__ void for_each_candidate(const Sorted& S, const Grid& G, int start_base, int slot_lo,
+                                                   int slot_hi, d3 x, F&& f) {
   // own cell, clamped so that a particle flagged outside the grid cannot
   // index outside the cell table
-  int ci = min(max(ref_cell(me.x.x, G, 0) - G.off[0], 0), G.n[0] - 1);
-  int cj = min(max(ref_cell(me.x.y, G, 1) - G.off[1], 0), G.n[1] - 1);
-  int ck = min(max(ref_cell(me.x.z, G, 2) - G.off[2], 0), G.n[2] - 1);
-
-  // The 27 neighbour cells as 9 rows (dj, dk) of up to 3 x-adjacent cells:
-  // cells of one row are consecutive keys, so each row is one contiguous run
-  // of sorted slots.  Cells reached across a periodic x face are handled
-  // apart with their shift.
+  const int ci = min(max(ref_cell(x.x, G, 0) - G.off[0], 0), G.n[0] - 1);
+  const int cj = min(max(ref_cell(x.y, G, 1) - G.off[1], 0), G.n[1] - 1);
+  const int ck = min(max(ref_cell(x.z, G, 2) - G.off[2], 0), G.n[2] - 1);
   int i0 = ci - 1, i1 = ci + 1;
   bool wrap_lo = false, wrap_hi = false;
   if (G.periodic[0]) {
@@ -653,88 +644,110 @@ __device__ __forceinline__ void for_each_contact(const Particles& P, const Sorte
     i0 = max(i0, 0);
     i1 = min(i1, G.n[0] - 1);
   }
-  for (int dj = -1; dj <= 1; ++dj) {
-    int nj = cj + dj;
-    double sy = 0.;
+  // neighbour row r = 3 (dj + 1) + (dk + 1): wrapped cell indices, image
+  // codes on y and z, and whether the row exists
+  auto row_of = [&](int r, int& row, int& iy, int& iz) {
+    int nj = cj + r / 3 - 1, nk = ck + r % 3 - 1;
+    iy = 0;
+    iz = 0;
     if (G.periodic[1]) {
-      if (nj < 0) { nj += G.n[1]; sy = -G.L[1]; }
-      else if (nj >= G.n[1]) { nj -= G.n[1]; sy = G.L[1]; }
-    } else if (nj < 0 || nj >= G.n[1]) continue;
-    for (int dk = -1; dk <= 1; ++dk) {
-      int nk = ck + dk;
-      double sz = 0.;
-      if (G.periodic[2]) {
-        if (nk < 0) { nk += G.n[2]; sz = -G.L[2]; }
-        else if (nk >= G.n[2]) { nk -= G.n[2]; sz = G.L[2]; }
-      } else if (nk < 0 || nk >= G.n[2]) continue;
-      const int row = start_base + (nj * G.n[2] + nk) * G.n[0];
-      // segments: the unshifted run, then the cells across the periodic x faces
-      for (int seg = 0; seg < 3; ++seg) {
-        int c0, c1;
-        double sx = 0.;
-        if (seg == 0) {
-          c0 = row + i0;
-          c1 = row + i1 + 1;
-        } else if (seg == 1) {
-          if (!wrap_lo) continue;
-          c0 = row + G.n[0] - 1;
-          c1 = c0 + 1;
-          sx = -G.L[0];
-        } else {
-          if (!wrap_hi) continue;
-          c0 = row;
-          c1 = row + 1;
-          sx = G.L[0];
-        }
-        const int b = max(S.start[c0], slot_lo), e = min(S.start[c1], slot_hi);
-        for (int k2 = b; k2 < e; ++k2) {
-          if (k2 == k) continue;
-          const double* q2 = S.posr + 4 * (size_t)k2;
-          // periodic clone position x + vec (added only on the wrapped axes)
-          d3 xo{sx != 0. ? q2[0] + sx : q2[0], sy != 0. ? q2[1] + sy : q2[1], sz != 0. ? q2[2] + sz : q2[2]};
-          double ro = q2[3];
-          d3 AB = xo - me.x;
-          double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
-          double R = me.r + ro;
-          // conservative reject: s2 > R^2 (1 + 1e-12) implies sqrt(s2) - R > 0
-          if (s2 > R * R * 1.000000000001) continue;
-          double nab = sqrt(s2);
-          const int s2i = S.order[k2];
-          if (s2i < slot_lo || s2i >= slot_hi) continue;
-          const double co = P.crust[s2i];
-          d3 Pc, ov;
-          double d;
-          if (SHAPES && (mg->type != kShapeSphere ||
-                         (P.shape[s2i] >= 0 && P.shapes[P.shape[s2i]].type != kShapeSphere))) {
-            ConvexGeo og = convex_of(P, s2i, xo, cx);
-            int res = closest_convex(*mg, og, Pc, ov, d);
-            if (res == 0) continue;
-            if (res < 0) {
-              if (!recover_gjk(*mg, og, Pc, ov, d)) {
-                raise_error(st, kContactError, me.id, S.sid[k2]);
-                continue;
-              }
-              atomicAdd(&st->n_recovered, 1);
-            }
-          } else {
-            int res = sphere_sphere(me.x, me.r, me.c, AB, nab, ro, co, Pc, ov, d);
-            if (res == 0) continue;
-            if (res < 0) {
-              if (!recover_ss(me.x, me.r, me.c, xo, ro, co, Pc, ov, d)) {
-                raise_error(st, kContactError, me.id, S.sid[k2]);
-                continue;
-              }
-              atomicAdd(&st->n_recovered, 1);
-            }
-          }
-          if (!(d < 0.)) continue;
-          vis.pair(k2, s2i, xo, Pc, ov, d);
-        }
+      if (nj < 0) { nj += G.n[1]; iy = -1; }
+      else if (nj >= G.n[1]) { nj -= G.n[1]; iy = 1; }
+    } else if (nj < 0 || nj >= G.n[1]) return false;
+    if (G.periodic[2]) {
+      if (nk < 0) { nk += G.n[2]; iz = -1; }
+      else if (nk >= G.n[2]) { nk -= G.n[2]; iz = 1; }
+    } else if (nk < 0 || nk >= G.n[2]) return false;
+    row = start_base + (nj * G.n[2] + nk) * G.n[0];
+    return true;
+  };
+  int rb[9], re[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    int row, iy, iz;
+    const bool ok = row_of(r, row, iy, iz);
+    rb[r] = ok ? max(S.start[row + i0], slot_lo) : 0;
+    re[r] = ok ? min(S.start[row + i1 + 1], slot_hi) : 0;
+  }
+  for (int r = 0; r < 9; ++r) {
+    int row, iy, iz;
+    if (!row_of(r, row, iy, iz)) continue;
+    // the row's unshifted run, then its cells across the periodic x faces
+    // (one call site of f, so that the visitor is inlined once)
+    for (int seg = 0; seg < 3; ++seg) {
+      int b, e, ix;
+      if (seg == 0) {
+        b = rb[r];
+        e = re[r];
+        ix = 0;
+      } else {
+        if (seg == 1 ? !wrap_lo : !wrap_hi) continue;
+        const int c0 = seg == 1 ? row + G.n[0] - 1 : row;
+        b = max(S.start[c0], slot_lo);
+        e = min(S.start[c0 + 1], slot_hi);
+        ix = seg == 1 ? -1 : 1;
       }
+      for (int k2 = b; k2 < e; ++k2) f(k2, ix, iy, iz);
     }
   }
+}
 
-  // obstacles (SimpleObstacle::InterAction, SimpleObstacle.cpp:222-271)
+// Narrow phase of one neighbour candidate (sorted slot k2, periodic image
+// (ix, iy, iz)) and, for a contact (overlap < 0), the visitor.
+// Particle::InterAction (Particle.cpp:477-545) with the ContactError
+// recovery of RigidBodyWithCrust.cpp:534-590.
+template <class V, bool SHAPES>
+__device__ __forceinline__ void visit_candidate(const Particles& P, const Sorted& S, const Grid& G, DevStatus* st,
+                                                int slot_lo, int slot_hi, const Self& me, V& vis,
+                                                const ConvexGeo* mg, int cx, int k2, int ix, int iy, int iz) {
+  const double* q2 = S.posr + 4 * (size_t)k2;
+  // periodic clone position x + vec (added only on the wrapped axes)
+  d3 xo{ix ? q2[0] + image_shift(G, 0, ix) : q2[0], iy ? q2[1] + image_shift(G, 1, iy) : q2[1],
+        iz ? q2[2] + image_shift(G, 2, iz) : q2[2]};
+  double ro = q2[3];
+  d3 AB = xo - me.x;
+  double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
+  double R = me.r + ro;
+  // conservative reject: s2 > R^2 (1 + 1e-12) implies sqrt(s2) - R > 0
+  if (s2 > R * R * 1.000000000001) return;
+  double nab = sqrt(s2);
+  const int s2i = S.order[k2];
+  if (s2i < slot_lo || s2i >= slot_hi) return;
+  const double co = P.crust[s2i];
+  d3 Pc, ov;
+  double d;
+  if (SHAPES && (mg->type != kShapeSphere ||
+                 (P.shape[s2i] >= 0 && P.shapes[P.shape[s2i]].type != kShapeSphere))) {
+    ConvexGeo og = convex_of(P, s2i, xo, cx);
+    int res = closest_convex(*mg, og, Pc, ov, d);
+    if (res == 0) return;
+    if (res < 0) {
+      if (!recover_gjk(*mg, og, Pc, ov, d)) {
+        raise_error(st, kContactError, me.id, S.sid[k2]);
+        return;
+      }
+      atomicAdd(&st->n_recovered, 1);
+    }
+  } else {
+    int res = sphere_sphere(me.x, me.r, me.c, AB, nab, ro, co, Pc, ov, d);
+    if (res == 0) return;
+    if (res < 0) {
+      if (!recover_ss(me.x, me.r, me.c, xo, ro, co, Pc, ov, d)) {
+        raise_error(st, kContactError, me.id, S.sid[k2]);
+        return;
+      }
+      atomicAdd(&st->n_recovered, 1);
+    }
+  }
+  if (!(d < 0.)) return;
+  vis.pair(k2, s2i, xo, Pc, ov, d);
+}
+
+// Particle-obstacle contacts (SimpleObstacle::InterAction,
+// SimpleObstacle.cpp:222-271).
+template <class V, bool SHAPES>
+__device__ __forceinline__ void visit_obstacles(const BoxObstacle* __restrict__ obs, int nobs, DevStatus* st,
+                                                const Self& me, V& vis, const ConvexGeo* mg) {
   for (int o = 0; o < nobs; ++o) {
     const BoxObstacle& B = obs[o];
     bool probable = fabs(B.aabb_c[0] - me.x.x) <= B.aabb_e[0] + me.r &&
@@ -768,6 +781,62 @@ __device__ __forceinline__ void for_each_contact(const Particles& P, const Sorte
     if (!(d < 0.)) continue;
     vis.wall(o, Pc, ov, d);
   }
+}
+
+// Enumerates the contacts (overlap < 0) of one particle (sorted slot k) with
+// its 27 neighbour cells and the obstacles -- LinkedCell::ComputeForces +
+// Particle::InterAction + SimpleObstacle::InterAction (LinkedCell.cpp:1019-1103,
+// Particle.cpp:477-545, SimpleObstacle.cpp:222-271) -- and hands each one to
+// the visitor:
+//   vis.pair(k2, s2i, xo, Pc, ov, d)   neighbour in sorted slot k2, storage
+//                                      s2i, seen at position xo (periodic image)
+//   vis.wall(o, Pc, ov, d)             obstacle o
+// with the contact point Pc, overlap vector ov and overlap distance d < 0 of
+// RigidBodyWithCrust::ClosestPoint (body A = this particle).  start_base
+// offsets the cell-start table; [slot_lo, slot_hi) bounds the sorted slots and
+// storage indices of the particle's system (one system: 0 and [0, N)).
+// With SHAPES the system holds non-spherical particles: mg describes this
+// particle's body (its shape, basis and crust) and cx selects the basis
+// buffer of the neighbours.
+template <class V, bool SHAPES = false>
+__device__ __forceinline__ void for_each_contact(const Particles& P, const Sorted& S, const Grid& G,
+                                                 const BoxObstacle* __restrict__ obs, int nobs, DevStatus* st,
+                                                 int start_base, int slot_lo, int slot_hi, int k, const Self& me,
+                                                 V& vis, const ConvexGeo* mg = nullptr, int cx = 0) {
+  for_each_candidate(S, G, start_base, slot_lo, slot_hi, me.x, [&](int k2, int ix, int iy, int iz) {
+    if (k2 != k) visit_candidate<V, SHAPES>(P, S, G, st, slot_lo, slot_hi, me, vis, mg, cx, k2, ix, iy, iz);
+  });
+  visit_obstacles<V, SHAPES>(obs, nobs, st, me, vis, mg);
+}
+
+// Candidate entry of a neighbour list: sorted slot in the low 27 bits,
+// periodic image code (ix+1) + 3(iy+1) + 9(iz+1) in the high 5.
+constexpr int kNbrSlotBits = kNbrSlotBitsHost;
+constexpr unsigned kNbrSlotMask = (1u << kNbrSlotBits) - 1u;
+__device__ __forceinline__ unsigned nbr_entry(int k2, int ix, int iy, int iz) {
+  return (unsigned)k2 | ((unsigned)((ix + 1) + 3 * (iy + 1) + 9 * (iz + 1)) << kNbrSlotBits);
+}
+
+// The same contacts as for_each_contact, from the candidate list written by
+// k_detect for this step (same candidates, same order).  A particle whose
+// candidates overflowed the list is swept in full.
+template <class V, bool SHAPES = false>
+__device__ __forceinline__ void for_each_listed(const Particles& P, const Sorted& S, const Grid& G,
+                                                const BoxObstacle* __restrict__ obs, int nobs, DevStatus* st,
+                                                int slot_lo, int slot_hi, int k, const Self& me, V& vis,
+                                                const NbrList& L, const ConvexGeo* mg = nullptr, int cx = 0) {
+  const int c = L.count[k];
+  if (c > L.cap) {
+    for_each_contact<V, SHAPES>(P, S, G, obs, nobs, st, 0, slot_lo, slot_hi, k, me, vis, mg, cx);
+    return;
+  }
+  for (int i = 0; i < c; ++i) {
+    const unsigned e = L.ent[(size_t)i * L.n + k];
+    const int code = (int)(e >> kNbrSlotBits);
+    visit_candidate<V, SHAPES>(P, S, G, st, slot_lo, slot_hi, me, vis, mg, cx, (int)(e & kNbrSlotMask),
+                               code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1);
+  }
+  visit_obstacles<V, SHAPES>(obs, nobs, st, me, vis, mg);
 }
 
 // Hooke-dashpot-Coulomb force and torque gathered on the particle
@@ -843,10 +912,14 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
                                                const HookeModel* __restrict__ models, const StepParams& sp,
                                                DevStatus* st, int* pairs, int start_base, int slot_lo, int slot_hi,
                                                int cv, int k, const Self& me, d3& F, d3& T,
-                                               const ConvexGeo* mg = nullptr, int cx = 0) {
+                                               const ConvexGeo* mg = nullptr, int cx = 0,
+                                               const NbrList* L = nullptr) {
   ForceVisitor<MIXED> vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
-  for_each_contact<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis,
-                                                mg, cx);
+  if (L && L->count)
+    for_each_listed<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx);
+  else
+    for_each_contact<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis,
+                                                  mg, cx);
   F = vis.F;
   T = vis.T;
   if (MIXED) {

@@ -7,7 +7,10 @@
 //   k_scatter            storage index to (cell, arrival slot)
 //   k_canon              per cell: order by particle id, gather (x,y,z,r) into
 //                        the sorted tile array, reset the counts
-//   k_forces<MODE>       one thread per sorted particle: 27-cell sweep with
+//   k_detect             one thread per sorted particle: 27-cell sweep with
+//                        minimum-image shifts, neighbour candidates -> list
+//   k_forces<MODE>       one thread per sorted particle: the listed (or, with
+//                        no list, the 27-cell) candidates with
 //                        minimum-image shifts, sphere-sphere / sphere-box
 //                        closest points, Hooke-dashpot-Coulomb force and
 //                        torque gathered in registers (no atomics), then the
@@ -182,6 +185,36 @@ __global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells
 }
 
 // ---------------------------------------------------------------------------
+// Broad phase of one step (LinkedCell::ComputeForces, LinkedCell.cpp:1019-1103):
+// every neighbour candidate of each sorted particle that passes the
+// conservative distance reject of the narrow phase, in the order the force
+// sweep visits them, written to the candidate list.  Holding only the
+// particle's position and the nine row ranges, the kernel keeps few
+// registers live, so many waves per SIMD overlap its dependent loads; the
+// force sweep then reads a short list instead of 27 cells.
+__global__ __launch_bounds__(256) void k_detect(Sorted S, Grid G, int n, NbrList L) {
+  const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* pr = S.posr + 4 * (size_t)k;
+  const d3 x{pr[0], pr[1], pr[2]};
+  const double r = pr[3];
+  int c = 0;
+  for_each_candidate(S, G, 0, 0, n, x, [&](int k2, int ix, int iy, int iz) {
+    if (k2 == k) return;
+    const double* q2 = S.posr + 4 * (size_t)k2;
+    d3 xo{ix ? q2[0] + image_shift(G, 0, ix) : q2[0], iy ? q2[1] + image_shift(G, 1, iy) : q2[1],
+          iz ? q2[2] + image_shift(G, 2, iz) : q2[2]};
+    d3 AB = xo - x;
+    double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
+    double R = r + q2[3];
+    if (s2 > R * R * 1.000000000001) return;
+    if (c < L.cap) L.ent[(size_t)c * n + k] = nbr_entry(k2, ix, iy, iz);
+    ++c;
+  });
+  L.count[k] = c;
+}
+
+// ---------------------------------------------------------------------------
 // Force sweep.  MODE: 0 forces only; 1 forces + second half kick; 2 forces +
 // second half kick + next step's Move (LeapFrog) or forces + Move(dt, dt)
 // (the other schemes).
@@ -190,7 +223,7 @@ enum { kForcesOnly = 0, kForcesKick = 1, kFused = 2 };
 template <int MODE, bool MIXED, bool SHAPES>
 __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
                                                 const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
-                                                int* pairs) {
+                                                int* pairs, NbrList L) {
   const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= sp.n) return;
   const int cx = sp.cur & 1, cv = (sp.cur >> 1) & 1;
@@ -215,7 +248,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   ConvexGeo mg;
   if (SHAPES) mg = convex_of(P, s, me.x, cx);
   sweep_particle<MIXED, SHAPES>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
-                                cx);
+                                cx, &L);
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
   if (!(MODE == kFused && sp.scheme == kLeapFrog)) {
@@ -336,27 +369,33 @@ int scan_blocks(int ncells) { return nblk(ncells, kScanTile); }
 template <int MODE, bool MIXED>
 static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
                             const HookeModel* models, const StepParams& sp, DevStatus* st, int* pairs,
-                            hipStream_t stream) {
+                            const NbrList& L, hipStream_t stream) {
   if (P.shape)
     hipLaunchKernelGGL((k_forces<MODE, MIXED, true>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs);
+                       models, sp, st, pairs, L);
   else
     hipLaunchKernelGGL((k_forces<MODE, MIXED, false>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs);
+                       models, sp, st, pairs, L);
+}
+
+hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_detect, dim3(nblk(n, 256)), dim3(256), 0, stream, S, G, n, L);
+  return hipGetLastError();
 }
 
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
                          const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,
-                         int* pairs, hipStream_t stream) {
+                         int* pairs, const NbrList& L, hipStream_t stream) {
   if (sp.n == 0) return hipSuccess;
   if (mixed) {
-    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, true>(P, S, G, obs, models, sp, st, pairs, stream);
-    else if (mode == kForcesKick) launch_forces_t<kForcesKick, true>(P, S, G, obs, models, sp, st, pairs, stream);
-    else launch_forces_t<kFused, true>(P, S, G, obs, models, sp, st, pairs, stream);
+    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, true>(P, S, G, obs, models, sp, st, pairs, L, stream);
+    else if (mode == kForcesKick) launch_forces_t<kForcesKick, true>(P, S, G, obs, models, sp, st, pairs, L, stream);
+    else launch_forces_t<kFused, true>(P, S, G, obs, models, sp, st, pairs, L, stream);
   } else {
-    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, false>(P, S, G, obs, models, sp, st, pairs, stream);
-    else if (mode == kForcesKick) launch_forces_t<kForcesKick, false>(P, S, G, obs, models, sp, st, pairs, stream);
-    else launch_forces_t<kFused, false>(P, S, G, obs, models, sp, st, pairs, stream);
+    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, false>(P, S, G, obs, models, sp, st, pairs, L, stream);
+    else if (mode == kForcesKick) launch_forces_t<kForcesKick, false>(P, S, G, obs, models, sp, st, pairs, L, stream);
+    else launch_forces_t<kFused, false>(P, S, G, obs, models, sp, st, pairs, L, stream);
   }
   return hipGetLastError();
 }

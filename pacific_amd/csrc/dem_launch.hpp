@@ -20,8 +20,11 @@ hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n,
 int scan_blocks(int ncells);
 // zero n ints with a kernel (safe inside captured graphs, unlike hipMemsetAsync)
 hipError_t launch_clear(int* p, int n, hipStream_t stream);
+// neighbour candidates of every sorted particle into L
+hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream);
+// force sweep over the candidates of L (L.count == nullptr: 27-cell sweep)
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
                          const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,
-                         int* pairs, hipStream_t stream);
+                         int* pairs, const NbrList& L, hipStream_t stream);
 
 }  // namespace dem

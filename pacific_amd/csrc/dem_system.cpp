@@ -123,6 +123,14 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, track(this, &d_obs, obstacles.size()));
   HIPCHECK(this, track(this, &d_models, (size_t)kMaxMaterials * kMaxMaterials));
   HIPCHECK(this, track(this, &d_status, 1));
+  // slots of the candidate list are stored in 27 bits
+  if (nbr_cap > 0 && n > 0 && n < ((size_t)1 << dem::kNbrSlotBitsHost)) {
+    HIPCHECK(this, track(this, &d_nbr_count, n));
+    HIPCHECK(this, track(this, &d_nbr, (size_t)nbr_cap * n));
+  } else {
+    d_nbr_count = nullptr;
+    d_nbr = nullptr;
+  }
   pair_cap = (int)std::min<size_t>(16 * n + 4096, (size_t)1 << 28);
   HIPCHECK(this, track(this, &d_pairs, 2 * (size_t)pair_cap));
   // index arrays start zeroed so that every stored index is in range
@@ -278,9 +286,15 @@ int grains_gpu_system::forces(int mode, int first_step) {
   if (e) return e;
   StepParams sp = params();
   sp.first_step = first_step;
+  const dem::NbrList L = nbr_list();
+  if (L.count) {
+    if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
+    HIPCHECK(this, launch_detect(S, grid, nparticles, L, stream));
+    if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
+  }
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
   HIPCHECK(this, launch_forces(mode, cfg.precision == GRAINS_MIXED, P, S, grid, d_obs, d_models, sp, d_status,
-                               d_pairs, stream));
+                               d_pairs, L, stream));
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
   if (mode == kModeForcesKick) cur ^= 2;
   if (mode == kModeFused) cur ^= 3;
@@ -698,9 +712,11 @@ int grains_gpu_profile_steps(grains_gpu_system* s, int nsteps, double* out) {
   grains_gpu_system::Profile pr;
   pr.force_ev.resize(2 * (size_t)nsteps + 2);
   pr.bin_ev.resize(2 * (size_t)nsteps + 2);
+  pr.detect_ev.resize(2 * (size_t)nsteps + 2);
   int e = GRAINS_OK;
   for (auto& ev : pr.force_ev) HIPCHECK(s, hipEventCreate(&ev));
   for (auto& ev : pr.bin_ev) HIPCHECK(s, hipEventCreate(&ev));
+  for (auto& ev : pr.detect_ev) HIPCHECK(s, hipEventCreate(&ev));
   HIPCHECK(s, hipEventRecord(s->ev0, s->stream));
   s->prof = &pr;
   e = s->enqueue_steps(nsteps);
@@ -724,9 +740,17 @@ int grains_gpu_profile_steps(grains_gpu_system* s, int nsteps, double* out) {
     out[2] = (double)(pr.nforce / 2);
     out[3] = bsum;
     out[4] = (double)(pr.nbin / 2);
+    double dsum = 0.;
+    for (size_t i = 0; i + 1 < pr.ndetect; i += 2) {
+      HIPCHECK(s, hipEventElapsedTime(&t, pr.detect_ev[i], pr.detect_ev[i + 1]));
+      dsum += t;
+    }
+    out[5] = dsum;
+    out[6] = (double)(pr.ndetect / 2);
   }
   for (auto& ev : pr.force_ev) (void)hipEventDestroy(ev);
   for (auto& ev : pr.bin_ev) (void)hipEventDestroy(ev);
+  for (auto& ev : pr.detect_ev) (void)hipEventDestroy(ev);
   if (e) return e;
   return s->check_status();
 }
@@ -870,6 +894,13 @@ int grains_gpu_get_masses(grains_gpu_system* s, double* mass, double* inertia, d
     if (weight)
       for (int a = 0; a < 3; ++a) weight[3 * i + a] = s->cfg.gravity[a] * s->h_wfac[i];
   }
+  return GRAINS_OK;
+}
+
+int grains_gpu_set_neighbor_list(grains_gpu_system* s, int capacity) {
+  if (!s || capacity < 0) return GRAINS_ARGUMENT_ERROR;
+  if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "the neighbour list is sized before initialisation");
+  s->nbr_cap = capacity;
   return GRAINS_OK;
 }
 

@@ -66,6 +66,12 @@ struct grains_gpu_system {
   dem::HookeModel* d_models = nullptr;
   dem::DevStatus* d_status = nullptr;
   int* d_pairs = nullptr;
+  // per-step neighbour candidate list (0 = the force sweep searches the 27
+  // cells itself); set before initialisation
+  int nbr_cap = 32;
+  int* d_nbr_count = nullptr;
+  unsigned* d_nbr = nullptr;
+  dem::NbrList nbr_list() const { return dem::NbrList{d_nbr_count, d_nbr, nbr_cap, nparticles}; }
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<void*> allocations;
@@ -82,8 +88,8 @@ struct grains_gpu_system {
   int forces(int mode, int first_step);
   // optional per-launch HIP events (force sweeps and binning sequences)
   struct Profile {
-    std::vector<hipEvent_t> force_ev, bin_ev;
-    size_t nforce = 0, nbin = 0;
+    std::vector<hipEvent_t> force_ev, bin_ev, detect_ev;
+    size_t nforce = 0, nbin = 0, ndetect = 0;
   };
   Profile* prof = nullptr;
   int mark(std::vector<hipEvent_t>& evs, size_t& n);

@@ -105,6 +105,17 @@ struct Sorted {
   double* posr;    // [4N] x, y, z, radius by sorted slot
 };
 
+// Neighbour candidates of each sorted particle for one step, written by
+// k_detect and read by the force sweep: entry i of slot k at ent[i * n + k]
+// (consecutive slots are consecutive words).  count[k] may exceed cap; such
+// a particle is swept in full by the force kernel.
+constexpr int kNbrSlotBitsHost = 27;  // slot bits of a list entry
+struct NbrList {
+  int* count;     // [n]
+  unsigned* ent;  // [cap * n]
+  int cap, n;
+};
+
 // Device-side error / statistics block (one per system)
 struct DevStatus {
   int code;          // first error: 1 contact, 2 motion, 3 simulation

@@ -45,7 +45,7 @@ EXPORTS = [
     "grains_gpu_contact_counts", "grains_gpu_add_shape", "grains_gpu_add_particles", "grains_gpu_shape_info",
     "grains_gpu_get_basis", "grains_gpu_set_basis", "grains_gpu_create_coupled", "grains_gpu_fluid_step",
     "grains_gpu_to_fluid", "grains_gpu_set_velocities", "grains_gpu_count_rigid_bodies",
-    "grains_gpu_time_interval", "grains_gpu_closest_point",
+    "grains_gpu_time_interval", "grains_gpu_closest_point", "grains_gpu_set_neighbor_list",
 ]
 
 
@@ -76,6 +76,7 @@ def lib():
         L.grains_gpu_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
         L.grains_gpu_get_masses.argtypes = [vp, dp, dp, dp]
         L.grains_gpu_set_record_pairs.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_set_neighbor_list.argtypes = [vp, ctypes.c_int]
         L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
         L.grains_gpu_num_particles.argtypes = [vp]
         L.grains_gpu_num_particles.restype = sz
@@ -349,11 +350,11 @@ class GrainsGPU:
 
     def profile_steps(self, n):
         """Runs n steps with HIP events around each kernel group; returns
-        dict(total_ms, force_ms, n_force, bin_ms, n_bin)."""
-        out = np.zeros(5)
+        dict(total_ms, force_ms, n_force, bin_ms, n_bin, detect_ms, n_detect)."""
+        out = np.zeros(7)
         self._check(lib().grains_gpu_profile_steps(self.h, int(n), _d(out)))
         return {"total_ms": out[0], "force_ms": out[1], "n_force": int(out[2]), "bin_ms": out[3],
-                "n_bin": int(out[4])}
+                "n_bin": int(out[4]), "detect_ms": out[5], "n_detect": int(out[6])}
 
     def synchronize(self):
         self._check(lib().grains_gpu_synchronize(self.h))
@@ -381,6 +382,11 @@ class GrainsGPU:
         m, I, wgt = np.zeros(n), np.zeros(n), np.zeros((n, 3))
         self._check(lib().grains_gpu_get_masses(self.h, _d(m), _d(I), _d(wgt)))
         return m, I, wgt
+
+    def neighbor_list(self, capacity):
+        """Candidates per particle of the per-step neighbour list (0: the force
+        sweep searches the 27 cells itself).  Before init() only."""
+        self._check(lib().grains_gpu_set_neighbor_list(self.h, int(capacity)))
 
     def record_pairs(self, on=True):
         self._check(lib().grains_gpu_set_record_pairs(self.h, 1 if on else 0))

@@ -14,11 +14,13 @@ def make_oracle(wl, scheme=0, fluid_density=0.0):
     return o
 
 
-def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True):
+def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None):
     from pacific_amd import GrainsGPU
     g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], fluid_density,
                   wl["dt"], scheme, precision)
     workloads.build(g, wl)
+    if nbr is not None:
+        g.neighbor_list(nbr)
     if record:
         g.record_pairs(True)
     g.init()
