@@ -621,15 +621,16 @@ __device__ __forceinline__ double image_shift(const Grid& G, int a, int c) {
   return c < 0 ? -G.L[a] : (c > 0 ? G.L[a] : 0.);
 }
 
-// Enumerates the neighbour candidates of a particle at x: every sorted slot
-// k2 of its 27 neighbour cells, with the periodic image (ix, iy, iz) under
-// which it is seen, in a fixed order (rows (dj, dk) of x-adjacent cells,
-// each row's unshifted run, then its cells across the periodic x faces).
-// Cells of one row are consecutive keys, so each row is one contiguous run
-// of sorted slots; the nine row ranges are loaded before any candidate.
+// Enumerates the neighbour candidates of a particle at x as runs of sorted
+// slots [b, e) of its 27 neighbour cells, each with the periodic image
+// (ix, iy, iz) under which its particles are seen, in a fixed order (rows
+// (dj, dk) of x-adjacent cells, each row's unshifted run, then its cells
+// across the periodic x faces).  Cells of one row are consecutive keys, so
+// each row is one contiguous run of sorted slots; the nine row ranges are
+// loaded before any candidate.
 template <class F>
-__device__ __forceinline__ void for_each_candidate(const Sorted& S, const Grid& G, int start_base, int slot_lo,
-                                                   int slot_hi, d3 x, F&& f) {
+__device__ __forceinline__ void for_each_run(const Sorted& S, const Grid& G, int start_base, int slot_lo,
+                                             int slot_hi, d3 x, F&& f) {
   // own cell, clamped so that a particle flagged outside the grid cannot
   // index outside the cell table
   const int ci = min(max(ref_cell(x.x, G, 0) - G.off[0], 0), G.n[0] - 1);
@@ -687,9 +688,18 @@ __device__ __forceinline__ void for_each_candidate(const Sorted& S, const Grid& 
         e = min(S.start[c0 + 1], slot_hi);
         ix = seg == 1 ? -1 : 1;
       }
-      for (int k2 = b; k2 < e; ++k2) f(k2, ix, iy, iz);
+      if (b < e) f(b, e, ix, iy, iz);
     }
   }
+}
+
+// The candidates of for_each_run one by one: f(k2, ix, iy, iz).
+template <class F>
+__device__ __forceinline__ void for_each_candidate(const Sorted& S, const Grid& G, int start_base, int slot_lo,
+                                                   int slot_hi, d3 x, F&& f) {
+  for_each_run(S, G, start_base, slot_lo, slot_hi, x, [&](int b, int e, int ix, int iy, int iz) {
+    for (int k2 = b; k2 < e; ++k2) f(k2, ix, iy, iz);
+  });
 }
 
 // Narrow phase of one neighbour candidate (sorted slot k2, periodic image
@@ -956,6 +966,7 @@ __device__ __forceinline__ void canon_cell(const Particles& P, const Sorted& S, 
       int s = ent[i];
       S.order[b + i] = s;
       S.sid[b + i] = ids[i];
+      if (S.inv) S.inv[s] = b + i;
       double* pr = S.posr + 4 * (size_t)(b + i);
       pr[0] = P.x[cx][0][s];
       pr[1] = P.x[cx][1][s];
@@ -979,6 +990,7 @@ __device__ __forceinline__ void canon_cell(const Particles& P, const Sorted& S, 
     }
     for (int i = 0; i < cnt; ++i) {
       int s = S.order[b + i];
+      if (S.inv) S.inv[s] = b + i;
       double* pr = S.posr + 4 * (size_t)(b + i);
       pr[0] = P.x[cx][0][s];
       pr[1] = P.x[cx][1][s];

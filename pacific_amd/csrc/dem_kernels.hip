@@ -9,8 +9,9 @@
 //                        the sorted tile array, reset the counts
 //   k_detect             one thread per sorted particle: 27-cell sweep with
 //                        minimum-image shifts, neighbour candidates -> list
-//   k_forces<MODE>       one thread per sorted particle: the listed (or, with
-//                        no list, the 27-cell) candidates with
+//   k_forces<MODE>       one thread per particle (storage order; sorted order
+//                        with non-spherical bodies): the listed (or, with no
+//                        list, the 27-cell) candidates with
 //                        minimum-image shifts, sphere-sphere / sphere-box
 //                        closest points, Hooke-dashpot-Coulomb force and
 //                        torque gathered in registers (no atomics), then the
@@ -199,6 +200,8 @@ __global__ __launch_bounds__(256) void k_detect(Sorted S, Grid G, int n, NbrList
   const d3 x{pr[0], pr[1], pr[2]};
   const double r = pr[3];
   int c = 0;
+  // one candidate at a time: at this register count eight waves per SIMD
+  // overlap the loads better than wider batches of loads per thread do
   for_each_candidate(S, G, 0, 0, n, x, [&](int k2, int ix, int iy, int iz) {
     if (k2 == k) return;
     const double* q2 = S.posr + 4 * (size_t)k2;
@@ -224,16 +227,22 @@ template <int MODE, bool MIXED, bool SHAPES>
 __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
                                                 const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
                                                 int* pairs, NbrList L) {
-  const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (k >= sp.n) return;
+  // Spheres: one thread per storage index, so that the particle's own state
+  // is read and written in contiguous runs; its sorted slot locates the cell
+  // data.  Non-spherical bodies: one thread per sorted slot, so that a wave
+  // holds neighbouring bodies (GJK dominates, and neighbours share branches).
+  const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= sp.n) return;
   const int cx = sp.cur & 1, cv = (sp.cur >> 1) & 1;
-  const int s = S.order[k];
-  if ((unsigned)s >= (unsigned)sp.n) return;  // only after an interrupted binning
+  const int s = SHAPES ? S.order[t] : t;
+  const int k = SHAPES ? t : S.inv[t];
+  // only after an interrupted binning
+  if ((unsigned)s >= (unsigned)sp.n || (unsigned)k >= (unsigned)sp.n) return;
   const double* pr = S.posr + 4 * (size_t)k;
   Self me;
   me.x = d3{pr[0], pr[1], pr[2]};
   me.r = pr[3];
-  me.id = S.sid[k];
+  me.id = P.id[s];
   me.v = ld3(P.v[cv], s);
   me.w = ld3(P.w[cv], s);
   me.m = P.mass[s];

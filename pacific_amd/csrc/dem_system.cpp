@@ -108,6 +108,7 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, track(this, &S.tmp_order, n));
   HIPCHECK(this, track(this, &S.order, n));
   HIPCHECK(this, track(this, &S.sid, n));
+  HIPCHECK(this, track(this, &S.inv, n));
   HIPCHECK(this, track(this, &S.posr, 4 * n));
 
   if (has_shapes()) {
@@ -137,6 +138,7 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, hipMemsetAsync(S.tmp_order, 0, sizeof(int) * n, stream));
   HIPCHECK(this, hipMemsetAsync(S.order, 0, sizeof(int) * n, stream));
   HIPCHECK(this, hipMemsetAsync(S.sid, 0, sizeof(int) * n, stream));
+  HIPCHECK(this, hipMemsetAsync(S.inv, 0, sizeof(int) * n, stream));
   HIPCHECK(this, hipMemsetAsync(S.posr, 0, sizeof(double) * 4 * n, stream));
   HIPCHECK(this, hipMemsetAsync(S.start, 0, sizeof(int) * ((size_t)grid.ncells + 1), stream));
   HIPCHECK(this, hipMemsetAsync(S.count, 0, sizeof(int) * (size_t)grid.ncells, stream));

@@ -102,6 +102,7 @@ struct Sorted {
   int* tmp_order;  // [N] storage index by (cell, slot)
   int* order;      // [N] storage index by (cell, id)
   int* sid;        // [N] particle id by sorted slot
+  int* inv;        // [N] sorted slot by storage index (nullptr: not kept)
   double* posr;    // [4N] x, y, z, radius by sorted slot
 };
 
