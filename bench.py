@@ -40,14 +40,15 @@ def log(*a):
 
 
 NBR_CAP = 32  # neighbour candidates per particle (0: the force sweep searches the 27 cells)
+SKIN = -1.0  # Verlet skin (< 0: the library default, a tenth of the cell size; 0: per-step list)
 
 
-def build_gpu(wl, precision):
+def build_gpu(wl, precision, skin=None):
     from pacific_amd import GrainsGPU, workloads
     g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], 0.0, wl["dt"],
                   0, precision)
     workloads.build(g, wl)
-    g.neighbor_list(NBR_CAP)
+    g.neighbor_list(NBR_CAP, SKIN if skin is None else skin)
     g.init()
     return g
 
@@ -109,7 +110,7 @@ def adjoint_leg(args):
     500-step forward run; wall clock around synchronised calls, both sides."""
     from pacific_amd import FP64, workloads
     wl = workloads.inclined_plane()
-    g = build_gpu(wl, FP64)
+    g = build_gpu(wl, FP64, skin=0.0)  # the adjoint replays the per-step neighbour search
     s0 = g.state()
     n, K = args.adjoint_steps, 50
     m, _, _ = g.masses()
@@ -199,6 +200,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--nbr-cap", type=int, default=32,
                     help="neighbour candidates per particle of the per-step list (0: 27-cell sweep in the force kernel)")
+    ap.add_argument("--skin", type=float, default=-1.0,
+                    help="Verlet skin in m (< 0: a tenth of the cell size; 0: candidate list rebuilt every step)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost-ratio leg")
@@ -212,8 +215,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
-    global NBR_CAP
+    global NBR_CAP, SKIN
     NBR_CAP = args.nbr_cap
+    SKIN = args.skin
 
     if args.calibrate_copy:
         # a plain copy of known size: the rocprofv3 FETCH_SIZE / WRITE_SIZE of
@@ -262,8 +266,10 @@ def main():
     detect_ms = prof["detect_ms"] / max(prof["n_detect"], 1) if prof["n_detect"] else 0.0
     b_alg = (B_ALG[args.precision] + B_PAIR * cbar) * args.n
     achieved = b_alg / (force_ms / 1e3) / 1e9
+    skin, builds = g.neighbor_stats()
     log(f"bench: force sweep {force_ms * 1e3:.1f} us/launch, neighbour search {detect_ms * 1e3:.1f} us/launch, "
-        f"binning {bin_ms * 1e3:.1f} us/step, contacts/particle {cbar:.4f}")
+        f"binning {bin_ms * 1e3:.1f} us/step, contacts/particle {cbar:.4f}, Verlet skin {skin:.3g} m, "
+        f"{builds} list builds in {args.warmup + args.steps + args.profile_steps} steps")
 
     extra = {}
     if not args.no_fp64 and prec != FP64:
@@ -303,7 +309,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"k_forces<fused,{args.precision}>", "kernel_ms": force_ms,
                      "bytes_alg_per_launch": b_alg, "contacts_per_particle": cbar, "binning_ms": bin_ms,
-                     "neighbour_search_ms": detect_ms},
+                     "neighbour_search_ms": detect_ms, "verlet_skin_m": skin, "verlet_builds": builds},
     }
     out.update(extra)
     if not args.no_cpu:

@@ -175,12 +175,22 @@ int grains_gpu_set_state(grains_gpu_system* sys, const double* x, const double* 
  * radius and density (Particle.cpp:157-166, 413-418). */
 int grains_gpu_get_masses(grains_gpu_system* sys, double* mass, double* inertia, double* weight);
 
-/* Capacity of the per-step neighbour candidate list (default 32 per
- * particle; 0 = none: the force sweep searches the 27 cells itself).  The
- * broad phase of LinkedCell::ComputeForces (LinkedCell.cpp:1019-1103) then
- * runs as its own launch; forces are bit-identical either way.  Before
- * grains_gpu_initialize only. */
-int grains_gpu_set_neighbor_list(grains_gpu_system* sys, int capacity);
+/* Neighbour candidates of the broad phase of LinkedCell::ComputeForces
+ * (LinkedCell.cpp:1019-1103), before grains_gpu_initialize only.
+ * capacity: listed candidates per particle (default 32; a particle with more
+ * is searched in its cells instead); 0 = no list, the force sweep searches
+ * the 27 cells itself.  skin = 0: the list is rebuilt every step, after the
+ * linked-cell binning, and forces are bit-identical to the cell search.
+ * skin > 0 (default, skin < 0: a tenth of the cell size 2 Rmax factor;
+ * spheres only): a Verlet list of every pair closer than the contact
+ * distance plus the skin, binned on its own grid and rebuilt only once some
+ * particle has moved more than half the skin, so that the contacts of every
+ * step are among the listed pairs: the same contacts and per-contact
+ * arithmetic, contact sums in the list's order. */
+int grains_gpu_set_neighbor_list(grains_gpu_system* sys, int capacity, double skin);
+/* Skin of the Verlet list in use (0: none) and the number of list builds
+ * so far.  No counterpart in the reference. */
+int grains_gpu_neighbor_stats(grains_gpu_system* sys, double* skin, long* builds);
 
 /* Contact records of the last force evaluation: pairs (receiver id, partner
  * id), one per particle that received a contact force from the partner

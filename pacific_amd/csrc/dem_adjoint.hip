@@ -68,14 +68,15 @@ struct AdjVisitor {
     in.mu = hm.muc;
     return in;
   }
-  __device__ __forceinline__ void pair(int k2, int s2i, d3 xo, d3 Pc, d3 ov, double d) {
+  __device__ __forceinline__ void pair(int k2, int s2i, int pid2, d3 xo, double ro, d3 Pc, d3 ov, double d) {
+    (void)k2;
     (void)Pc;
     (void)ov;
     (void)d;
     const int mo = P.mat[s2i];
     const HookeModel& hm = models[me.mat * kMaxMaterials + mo];
     ContactIn in = base(hm);
-    in.r1 = S.posr[4 * (size_t)k2 + 3];
+    in.r1 = ro;
     in.v1 = ld3(P.v[cv], s2i);
     in.w1 = ld3(P.w[cv], s2i);
     in.m1 = P.mass[s2i];
@@ -86,7 +87,7 @@ struct AdjVisitor {
     acc.w0 += g.w0;
     // each pair is visited from both sides; its parameter derivative is
     // counted once
-    if (target(me.mat, mo) && me.id < S.sid[k2]) {
+    if (target(me.mat, mo) && me.id < pid2) {
       acc.kn += g.kn;
       acc.beta += g.beta;
       acc.mu += g.mu;
@@ -310,6 +311,10 @@ int grains_gpu_adjoint_forward(grains_gpu_system* s, int nsteps, int checkpoint_
   if (s->cfg.scheme != GRAINS_LEAPFROG) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates SecondOrderLeapFrog");
   if (s->cfg.precision != GRAINS_FP64) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates the FP64 step");
   if (s->has_shapes()) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates systems of spheres");
+  if (s->verlet)
+    return afail(s, GRAINS_CONFIG_ERROR,
+                 "the adjoint replays the per-step neighbour search: set the skin to 0 with "
+                 "grains_gpu_set_neighbor_list before initialisation");
   for (int m = 0; m < kMaxMaterials * kMaxMaterials; ++m)
     if (s->models[m].defined && s->models[m].kr != 0.)
       return afail(s, GRAINS_CONFIG_ERROR, "the adjoint does not differentiate rolling resistance (kr != 0)");

@@ -514,7 +514,7 @@ __device__ __forceinline__ Kin load_kin(const Particles& P, int s, int cx, int c
 // ParticleKinematics::setAngularVelocity (ParticleKinematics.cpp:197-203).
 // Then the cell key is computed and the arrival slot reserved.
 __device__ __forceinline__ void wrap_and_bin(Kin& k, const Grid& G, const Sorted& S, DevStatus* st, int s, int pid,
-                                             int cell_base = 0) {
+                                             int cell_base = 0, const VerletState* V = nullptr) {
   bool wrapped = false;
   double* xs[3] = {&k.x.x, &k.x.y, &k.x.z};
   int c[3];
@@ -540,6 +540,25 @@ __device__ __forceinline__ void wrap_and_bin(Kin& k, const Grid& G, const Sorted
     c[a] = gi;
   }
   if (wrapped) k.wb = to_body(k.q, k.w);
+  if (V && V->stamp) {
+    // a Verlet list is in use: no binning here (stamp < 0: the list is
+    // rebuilt next, nothing to check); the list goes stale once the
+    // particle is more than half the skin from where it was built (nearest
+    // periodic image, so that a role swap is no displacement)
+    if (V->stamp < 0) return;
+    double d2 = 0.;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double d = *xs[a] - V->xb[a][s];
+      if (G.periodic[a]) {
+        if (d > 0.5 * G.L[a]) d -= G.L[a];
+        else if (d < -0.5 * G.L[a]) d += G.L[a];
+      }
+      d2 += d * d;
+    }
+    if (d2 > V->half2) st->rebuild = V->stamp;
+    return;
+  }
   int key = cell_base + (c[1] * G.n[2] + c[2]) * G.n[0] + c[0];  // LinkedCell::getCellNumber order
   S.key[s] = key;
   S.slot[s] = atomicAdd(&S.count[key], 1);
@@ -702,10 +721,48 @@ __device__ __forceinline__ void for_each_candidate(const Sorted& S, const Grid& 
   });
 }
 
-// Narrow phase of one neighbour candidate (sorted slot k2, periodic image
-// (ix, iy, iz)) and, for a contact (overlap < 0), the visitor.
-// Particle::InterAction (Particle.cpp:477-545) with the ContactError
-// recovery of RigidBodyWithCrust.cpp:534-590.
+// Narrow phase of a neighbour candidate that passed the conservative
+// distance reject: particle s2i (storage index; sorted slot k2 when known,
+// id pid2, radius ro) seen at xo, AB = xo - x_self, nab = Norm(AB); for a
+// contact (overlap < 0), the visitor.  Particle::InterAction
+// (Particle.cpp:477-545) with the ContactError recovery of
+// RigidBodyWithCrust.cpp:534-590.
+template <class V, bool SHAPES>
+__device__ __forceinline__ void narrow_visit(const Particles& P, DevStatus* st, const Self& me, V& vis,
+                                             const ConvexGeo* mg, int cx, int k2, int s2i, int pid2, d3 xo,
+                                             double ro, d3 AB, double nab) {
+  const double co = P.crust[s2i];
+  d3 Pc, ov;
+  double d;
+  if (SHAPES && (mg->type != kShapeSphere ||
+                 (P.shape[s2i] >= 0 && P.shapes[P.shape[s2i]].type != kShapeSphere))) {
+    ConvexGeo og = convex_of(P, s2i, xo, cx);
+    int res = closest_convex(*mg, og, Pc, ov, d);
+    if (res == 0) return;
+    if (res < 0) {
+      if (!recover_gjk(*mg, og, Pc, ov, d)) {
+        raise_error(st, kContactError, me.id, pid2);
+        return;
+      }
+      atomicAdd(&st->n_recovered, 1);
+    }
+  } else {
+    int res = sphere_sphere(me.x, me.r, me.c, AB, nab, ro, co, Pc, ov, d);
+    if (res == 0) return;
+    if (res < 0) {
+      if (!recover_ss(me.x, me.r, me.c, xo, ro, co, Pc, ov, d)) {
+        raise_error(st, kContactError, me.id, pid2);
+        return;
+      }
+      atomicAdd(&st->n_recovered, 1);
+    }
+  }
+  if (!(d < 0.)) return;
+  vis.pair(k2, s2i, pid2, xo, ro, Pc, ov, d);
+}
+
+// One neighbour candidate of the cell sweep (sorted slot k2, periodic image
+// (ix, iy, iz)).
 template <class V, bool SHAPES>
 __device__ __forceinline__ void visit_candidate(const Particles& P, const Sorted& S, const Grid& G, DevStatus* st,
                                                 int slot_lo, int slot_hi, const Self& me, V& vis,
@@ -723,34 +780,32 @@ __device__ __forceinline__ void visit_candidate(const Particles& P, const Sorted
   double nab = sqrt(s2);
   const int s2i = S.order[k2];
   if (s2i < slot_lo || s2i >= slot_hi) return;
-  const double co = P.crust[s2i];
-  d3 Pc, ov;
-  double d;
-  if (SHAPES && (mg->type != kShapeSphere ||
-                 (P.shape[s2i] >= 0 && P.shapes[P.shape[s2i]].type != kShapeSphere))) {
-    ConvexGeo og = convex_of(P, s2i, xo, cx);
-    int res = closest_convex(*mg, og, Pc, ov, d);
-    if (res == 0) return;
-    if (res < 0) {
-      if (!recover_gjk(*mg, og, Pc, ov, d)) {
-        raise_error(st, kContactError, me.id, S.sid[k2]);
-        return;
-      }
-      atomicAdd(&st->n_recovered, 1);
+  narrow_visit<V, SHAPES>(P, st, me, vis, mg, cx, k2, s2i, S.sid[k2], xo, ro, AB, nab);
+}
+
+// Particle s2i at its current position (position buffer cx), seen at its
+// periodic image nearest to this particle: for any pair closer than half
+// the domain -- every pair in contact -- that is the clone position x + vec
+// of the cell sweep, bit for bit.
+template <class V, bool SHAPES>
+__device__ __forceinline__ void visit_current(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
+                                              V& vis, const ConvexGeo* mg, int cx, int s2i) {
+  d3 xo{P.x[cx][0][s2i], P.x[cx][1][s2i], P.x[cx][2][s2i]};
+  double* xs[3] = {&xo.x, &xo.y, &xo.z};
+  const double* ms[3] = {&me.x.x, &me.x.y, &me.x.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (G.periodic[a]) {
+      const double d = *xs[a] - *ms[a];
+      if (d > 0.5 * G.L[a]) *xs[a] = *xs[a] + image_shift(G, a, -1);
+      else if (d < -0.5 * G.L[a]) *xs[a] = *xs[a] + image_shift(G, a, 1);
     }
-  } else {
-    int res = sphere_sphere(me.x, me.r, me.c, AB, nab, ro, co, Pc, ov, d);
-    if (res == 0) return;
-    if (res < 0) {
-      if (!recover_ss(me.x, me.r, me.c, xo, ro, co, Pc, ov, d)) {
-        raise_error(st, kContactError, me.id, S.sid[k2]);
-        return;
-      }
-      atomicAdd(&st->n_recovered, 1);
-    }
-  }
-  if (!(d < 0.)) return;
-  vis.pair(k2, s2i, xo, Pc, ov, d);
+  const double ro = P.rad[s2i];
+  d3 AB = xo - me.x;
+  double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
+  double R = me.r + ro;
+  if (s2 > R * R * 1.000000000001) return;
+  narrow_visit<V, SHAPES>(P, st, me, vis, mg, cx, -1, s2i, P.id[s2i], xo, ro, AB, sqrt(s2));
 }
 
 // Particle-obstacle contacts (SimpleObstacle::InterAction,
@@ -849,6 +904,31 @@ __device__ __forceinline__ void for_each_listed(const Particles& P, const Sorted
   visit_obstacles<V, SHAPES>(obs, nobs, st, me, vis, mg);
 }
 
+// The contacts of particle s from the Verlet list (neighbours by storage
+// index, in the order of the cell sweep at the last build); a particle
+// whose candidates overflowed the list enumerates the same cells of the
+// list's grid around its build position, in the same order.
+template <class V, bool SHAPES = false>
+__device__ __forceinline__ void for_each_verlet(const Particles& P, const Sorted& S, const Grid& G,
+                                                const VerletState& VS, const BoxObstacle* __restrict__ obs, int nobs,
+                                                DevStatus* st, int n, int s, const Self& me, V& vis,
+                                                const NbrList& L, const ConvexGeo* mg = nullptr, int cx = 0) {
+  const int c = L.count[s];
+  if (c <= L.cap) {
+    for (int i = 0; i < c; ++i) {
+      const unsigned s2i = L.ent[(size_t)i * L.n + s];
+      if (s2i < (unsigned)n) visit_current<V, SHAPES>(P, G, st, me, vis, mg, cx, (int)s2i);
+    }
+  } else {
+    const d3 xb{VS.xb[0][s], VS.xb[1][s], VS.xb[2][s]};
+    for_each_candidate(S, VS.Gv, 0, 0, n, xb, [&](int k2, int, int, int) {
+      const int s2i = S.order[k2];
+      if (s2i != s && (unsigned)s2i < (unsigned)n) visit_current<V, SHAPES>(P, G, st, me, vis, mg, cx, s2i);
+    });
+  }
+  visit_obstacles<V, SHAPES>(obs, nobs, st, me, vis, mg);
+}
+
 // Hooke-dashpot-Coulomb force and torque gathered on the particle
 // (HookeContactForceModel.cpp:121-163, Torsor.cpp:184-190); F enters holding
 // the weight.  MIXED evaluates the model in FP32 on lever arms.
@@ -866,7 +946,9 @@ struct ForceVisitor {
   d3 F, T;
   f3 Fc, Tc;
 
-  __device__ __forceinline__ void pair(int k2, int s2i, d3 xo, d3 Pc, d3 ov, double d) {
+  __device__ __forceinline__ void pair(int k2, int s2i, int pid2, d3 xo, double ro, d3 Pc, d3 ov, double d) {
+    (void)k2;
+    (void)ro;
     (void)d;
     const HookeModel& hm = models[me.mat * kMaxMaterials + P.mat[s2i]];
     d3 vo = ld3(P.v[cv], s2i), wo = ld3(P.w[cv], s2i);
@@ -887,7 +969,7 @@ struct ForceVisitor {
       T += cross(Pc - me.x, f);
       if (hm.kr != 0.) T += dM * 1.;
     }
-    record_pair(sp, st, pairs, me.id, S.sid[k2], slip);
+    record_pair(sp, st, pairs, me.id, pid2, slip);
   }
 
   __device__ __forceinline__ void wall(int o, d3 Pc, d3 ov, double d) {
@@ -923,9 +1005,12 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
                                                DevStatus* st, int* pairs, int start_base, int slot_lo, int slot_hi,
                                                int cv, int k, const Self& me, d3& F, d3& T,
                                                const ConvexGeo* mg = nullptr, int cx = 0,
-                                               const NbrList* L = nullptr) {
+                                               const NbrList* L = nullptr, const VerletState* VS = nullptr,
+                                               int s = -1) {
   ForceVisitor<MIXED> vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
-  if (L && L->count)
+  if (VS && VS->stamp)
+    for_each_verlet<ForceVisitor<MIXED>, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx);
+  else if (L && L->count)
     for_each_listed<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx);
   else
     for_each_contact<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis,

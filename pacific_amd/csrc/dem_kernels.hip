@@ -33,7 +33,7 @@ namespace dem {
 // First half kick + drift over storage indices (in place): Grains.cpp:353
 // moveParticlesAndObstacles(dt/2, dt, dt) -> AllComponents::Move.
 __global__ __launch_bounds__(256) void k_move(Particles P, Sorted S, Grid G, StepParams sp, DevStatus* st,
-                                              double dtv, double dtx) {
+                                              double dtv, double dtx, VerletState V) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= sp.n) return;
   const int cx = sp.cur & 1, cv = (sp.cur >> 1) & 1;
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_move(Particles P, Sorted S, Grid G, Ste
   }
   int pid = P.id[s];
   if (depl > P.crust[s]) raise_error(st, kMotionError, pid, 0);  // Particle.cpp:444-456
-  wrap_and_bin(k, G, S, st, s, pid);
+  wrap_and_bin(k, G, S, st, s, pid, 0, &V);
   st3(P.x[cx], s, k.x);
   st3(P.v[cv], s, k.v);
   st3(P.w[cv], s, k.w);
@@ -84,13 +84,15 @@ __global__ __launch_bounds__(256) void k_move(Particles P, Sorted S, Grid G, Ste
 // Cell key and arrival slot of the current positions, without motion (used
 // after initialisation and after a state upload).  Positions outside the
 // periodic box are brought back first.
-__global__ __launch_bounds__(256) void k_key(Particles P, Sorted S, Grid G, StepParams sp, DevStatus* st) {
+__global__ __launch_bounds__(256) void k_key(Particles P, Sorted S, Grid G, StepParams sp, DevStatus* st,
+                                             VerletState V) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= sp.n) return;
   const int cx = sp.cur & 1, cv = (sp.cur >> 1) & 1;
   Kin k = load_kin(P, s, cx, cv);
   d3 x0 = k.x;
-  wrap_and_bin(k, G, S, st, s, P.id[s]);
+  // with a Verlet list the periodic wrap only (the list is rebuilt next)
+  wrap_and_bin(k, G, S, st, s, P.id[s], 0, &V);
   if (k.x.x != x0.x || k.x.y != x0.y || k.x.z != x0.z) {
     st3(P.x[cx], s, k.x);
     st3(P.wb, s, k.wb);
@@ -121,7 +123,9 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total)
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* __restrict__ count, int ncells,
-                                                              int* __restrict__ block_sums) {
+                                                              int* __restrict__ block_sums, const int* gate,
+                                                              int stamp) {
+  if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
   __shared__ int lds[kScanThreads];
   int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int sum = 0;
@@ -132,7 +136,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* __restr
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_top(int* __restrict__ block_sums, int nblocks) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(int* __restrict__ block_sums, int nblocks, const int* gate,
+                                                           int stamp) {
+  if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
   __shared__ int lds[kScanThreads];
   int carry = 0;
   for (int b0 = 0; b0 < nblocks; b0 += kScanThreads) {
@@ -147,7 +153,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(int* __restrict__ blo
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int* __restrict__ count, int ncells,
                                                             const int* __restrict__ block_sums,
-                                                            int* __restrict__ start, int n) {
+                                                            int* __restrict__ start, int n, const int* gate,
+                                                            int stamp) {
+  if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
   __shared__ int lds[kScanThreads];
   int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int vals[kScanItems];
@@ -168,7 +176,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int* __restric
 }
 
 // Storage index -> (cell start + arrival slot)
-__global__ __launch_bounds__(256) void k_scatter(Sorted S, int n) {
+__global__ __launch_bounds__(256) void k_scatter(Sorted S, int n, const int* gate, int stamp) {
+  if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int dst = S.start[S.key[s]] + S.slot[s];
@@ -178,7 +187,9 @@ __global__ __launch_bounds__(256) void k_scatter(Sorted S, int n) {
 // One thread per cell: order the cell's particles by id (deterministic, and
 // independent of the atomic arrival order), gather the sorted tile arrays and
 // reset the count for the next step.
-__global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells, int cx, int n) {
+__global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells, int cx, int n, const int* gate,
+                                               int stamp) {
+  if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   S.count[c] = 0;
@@ -226,7 +237,7 @@ enum { kForcesOnly = 0, kForcesKick = 1, kFused = 2 };
 template <int MODE, bool MIXED, bool SHAPES>
 __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
                                                 const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
-                                                int* pairs, NbrList L) {
+                                                int* pairs, NbrList L, VerletState VS) {
   // Spheres: one thread per storage index, so that the particle's own state
   // is read and written in contiguous runs; its sorted slot locates the cell
   // data.  Non-spherical bodies: one thread per sorted slot, so that a wave
@@ -238,10 +249,16 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   const int k = SHAPES ? t : S.inv[t];
   // only after an interrupted binning
   if ((unsigned)s >= (unsigned)sp.n || (unsigned)k >= (unsigned)sp.n) return;
-  const double* pr = S.posr + 4 * (size_t)k;
   Self me;
-  me.x = d3{pr[0], pr[1], pr[2]};
-  me.r = pr[3];
+  if (VS.stamp) {
+    // Verlet list: the sorted arrays hold the positions of the last build
+    me.x = ld3(P.x[cx], s);
+    me.r = P.rad[s];
+  } else {
+    const double* pr = S.posr + 4 * (size_t)k;
+    me.x = d3{pr[0], pr[1], pr[2]};
+    me.r = pr[3];
+  }
   me.id = P.id[s];
   me.v = ld3(P.v[cv], s);
   me.w = ld3(P.w[cv], s);
@@ -257,7 +274,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   ConvexGeo mg;
   if (SHAPES) mg = convex_of(P, s, me.x, cx);
   sweep_particle<MIXED, SHAPES>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
-                                cx, &L);
+                                cx, &L, &VS, s);
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
   if (!(MODE == kFused && sp.scheme == kLeapFrog)) {
@@ -281,7 +298,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
     if (MODE == kFused) {
       double depl = lf_move_body(kin, gyro, F, T, me.m, cf, I, I3, 0.5 * sp.dt, sp.dt, mg.B);
       if (depl > me.c) raise_error(st, kMotionError, me.id, 0);
-      wrap_and_bin(kin, G, S, st, s, me.id);
+      wrap_and_bin(kin, G, S, st, s, me.id, 0, &VS);
       st3(P.x[cx ^ 1], s, kin.x);
       for (int j = 0; j < 9; ++j) P.B[cx ^ 1][j][s] = mg.B[j];
       P.q[0][s] = kin.q.v.x;
@@ -323,7 +340,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
     }
   }
   if (depl > me.c) raise_error(st, kMotionError, me.id, 0);
-  wrap_and_bin(kin, G, S, st, s, me.id);
+  wrap_and_bin(kin, G, S, st, s, me.id, 0, &VS);
   st3(P.x[cx ^ 1], s, kin.x);
   st3(P.v[nv], s, kin.v);
   st3(P.w[nv], s, kin.w);
@@ -339,9 +356,9 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 hipError_t launch_move(const Particles& P, const Sorted& S, const Grid& G, const StepParams& sp, DevStatus* st,
-                       double dtv, double dtx, hipStream_t stream) {
+                       double dtv, double dtx, const VerletState& V, hipStream_t stream) {
   if (sp.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_move, dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, sp, st, dtv, dtx);
+  hipLaunchKernelGGL(k_move, dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, sp, st, dtv, dtx, V);
   return hipGetLastError();
 }
 
@@ -356,20 +373,82 @@ hipError_t launch_clear(int* p, int n, hipStream_t stream) {
 }
 
 hipError_t launch_key(const Particles& P, const Sorted& S, const Grid& G, const StepParams& sp, DevStatus* st,
-                      hipStream_t stream) {
+                      const VerletState& V, hipStream_t stream) {
   if (sp.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_key, dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, sp, st);
+  hipLaunchKernelGGL(k_key, dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, sp, st, V);
   return hipGetLastError();
 }
 
-hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n, int cx, hipStream_t stream) {
+hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n, int cx, hipStream_t stream,
+                      const int* gate, int stamp) {
   int nb = nblk(G.ncells, kScanTile);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, S.block_sums, nb);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums, gate,
+                     stamp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, S.block_sums, nb, gate, stamp);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums,
-                     S.start, n);
-  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, stream, S, n);
-  hipLaunchKernelGGL(k_canon, dim3(nblk(G.ncells, 256)), dim3(256), 0, stream, P, S, G.ncells, cx, n);
+                     S.start, n, gate, stamp);
+  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, stream, S, n, gate, stamp);
+  hipLaunchKernelGGL(k_canon, dim3(nblk(G.ncells, 256)), dim3(256), 0, stream, P, S, G.ncells, cx, n, gate, stamp);
+  return hipGetLastError();
+}
+
+// Verlet list build: cell keys on the list's grid (and the build positions),
+// binning, then the candidates of every particle within the contact
+// distance plus the skin.  Gated on the stale flag unless stamp == 0.
+__global__ __launch_bounds__(256) void k_vkey(Particles P, Sorted S, VerletState V, int n, int cx, DevStatus* st,
+                                              int stamp) {
+  const int* gate = stamp ? &st->rebuild : nullptr;
+  if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) atomicAdd(&st->n_rebuild, 1);
+  if (s >= n) return;
+  const Grid& G = V.Gv;
+  const double x[3] = {P.x[cx][0][s], P.x[cx][1][s], P.x[cx][2][s]};
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    c[a] = min(max(ref_cell(x[a], G, a), 0), G.n[a] - 1);
+    V.xb[a][s] = x[a];
+  }
+  const int key = (c[1] * G.n[2] + c[2]) * G.n[0] + c[0];
+  S.key[s] = key;
+  S.slot[s] = atomicAdd(&S.count[key], 1);
+}
+
+__global__ __launch_bounds__(256) void k_vdetect(Sorted S, VerletState V, int n, NbrList L, const int* gate,
+                                                 int stamp) {
+  if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
+  const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int s = S.order[k];
+  if ((unsigned)s >= (unsigned)n) return;
+  const double* pr = S.posr + 4 * (size_t)k;
+  const d3 x{pr[0], pr[1], pr[2]};
+  const double r = pr[3] + V.skin;
+  int c = 0;
+  for_each_candidate(S, V.Gv, 0, 0, n, x, [&](int k2, int ix, int iy, int iz) {
+    if (k2 == k) return;
+    const double* q2 = S.posr + 4 * (size_t)k2;
+    d3 xo{ix ? q2[0] + image_shift(V.Gv, 0, ix) : q2[0], iy ? q2[1] + image_shift(V.Gv, 1, iy) : q2[1],
+          iz ? q2[2] + image_shift(V.Gv, 2, iz) : q2[2]};
+    d3 AB = xo - x;
+    double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
+    double R = r + q2[3];
+    if (s2 > R * R * 1.000000000001) return;
+    if (c < L.cap) L.ent[(size_t)c * n + s] = (unsigned)S.order[k2];
+    ++c;
+  });
+  L.count[s] = c;
+}
+
+hipError_t launch_verlet_build(const Particles& P, const Sorted& S, const VerletState& V, int n, int cx,
+                               DevStatus* st, const NbrList& L, int stamp, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vkey, dim3(nblk(n, 256)), dim3(256), 0, stream, P, S, V, n, cx, st, stamp);
+  const int* gate = stamp ? &st->rebuild : nullptr;
+  hipError_t e = launch_bin(P, S, V.Gv, n, cx, stream, gate, stamp);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_vdetect, dim3(nblk(n, 256)), dim3(256), 0, stream, S, V, n, L, gate, stamp);
   return hipGetLastError();
 }
 
@@ -378,13 +457,13 @@ int scan_blocks(int ncells) { return nblk(ncells, kScanTile); }
 template <int MODE, bool MIXED>
 static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
                             const HookeModel* models, const StepParams& sp, DevStatus* st, int* pairs,
-                            const NbrList& L, hipStream_t stream) {
+                            const NbrList& L, const VerletState& V, hipStream_t stream) {
   if (P.shape)
     hipLaunchKernelGGL((k_forces<MODE, MIXED, true>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs, L);
+                       models, sp, st, pairs, L, V);
   else
     hipLaunchKernelGGL((k_forces<MODE, MIXED, false>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs, L);
+                       models, sp, st, pairs, L, V);
 }
 
 hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream) {
@@ -395,16 +474,16 @@ hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L
 
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
                          const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,
-                         int* pairs, const NbrList& L, hipStream_t stream) {
+                         int* pairs, const NbrList& L, const VerletState& V, hipStream_t stream) {
   if (sp.n == 0) return hipSuccess;
   if (mixed) {
-    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, true>(P, S, G, obs, models, sp, st, pairs, L, stream);
-    else if (mode == kForcesKick) launch_forces_t<kForcesKick, true>(P, S, G, obs, models, sp, st, pairs, L, stream);
-    else launch_forces_t<kFused, true>(P, S, G, obs, models, sp, st, pairs, L, stream);
+    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, true>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
+    else if (mode == kForcesKick) launch_forces_t<kForcesKick, true>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
+    else launch_forces_t<kFused, true>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
   } else {
-    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, false>(P, S, G, obs, models, sp, st, pairs, L, stream);
-    else if (mode == kForcesKick) launch_forces_t<kForcesKick, false>(P, S, G, obs, models, sp, st, pairs, L, stream);
-    else launch_forces_t<kFused, false>(P, S, G, obs, models, sp, st, pairs, L, stream);
+    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, false>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
+    else if (mode == kForcesKick) launch_forces_t<kForcesKick, false>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
+    else launch_forces_t<kFused, false>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
   }
   return hipGetLastError();
 }

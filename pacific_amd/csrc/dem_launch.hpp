@@ -11,12 +11,18 @@ enum ForceMode { kModeForcesOnly = 0, kModeForcesKick = 1, kModeFused = 2 };
 // first half kick + drift (or a full Move of the explicit schemes) in place,
 // then cell key and arrival slot
 hipError_t launch_move(const Particles& P, const Sorted& S, const Grid& G, const StepParams& sp, DevStatus* st,
-                       double dtv, double dtx, hipStream_t stream);
+                       double dtv, double dtx, const VerletState& V, hipStream_t stream);
 // cell key and arrival slot of the current positions (no motion)
 hipError_t launch_key(const Particles& P, const Sorted& S, const Grid& G, const StepParams& sp, DevStatus* st,
-                      hipStream_t stream);
-// counts -> start, scatter, per-cell ordering; cx = position buffer to gather
-hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n, int cx, hipStream_t stream);
+                      const VerletState& V, hipStream_t stream);
+// counts -> start, scatter, per-cell ordering; cx = position buffer to gather;
+// with gate, only when *gate == stamp
+hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n, int cx, hipStream_t stream,
+                      const int* gate = nullptr, int stamp = 0);
+// Verlet list build (keys on its grid, binning, candidates); stamp > 0: only
+// if the list was found stale by the launch with that stamp
+hipError_t launch_verlet_build(const Particles& P, const Sorted& S, const VerletState& V, int n, int cx,
+                               DevStatus* st, const NbrList& L, int stamp, hipStream_t stream);
 int scan_blocks(int ncells);
 // zero n ints with a kernel (safe inside captured graphs, unlike hipMemsetAsync)
 hipError_t launch_clear(int* p, int n, hipStream_t stream);
@@ -25,6 +31,6 @@ hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L
 // force sweep over the candidates of L (L.count == nullptr: 27-cell sweep)
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
                          const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,
-                         int* pairs, const NbrList& L, hipStream_t stream);
+                         int* pairs, const NbrList& L, const VerletState& V, hipStream_t stream);
 
 }  // namespace dem

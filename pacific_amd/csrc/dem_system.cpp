@@ -124,6 +124,8 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, track(this, &d_obs, obstacles.size()));
   HIPCHECK(this, track(this, &d_models, (size_t)kMaxMaterials * kMaxMaterials));
   HIPCHECK(this, track(this, &d_status, 1));
+  if (verlet)
+    for (int a = 0; a < 3; ++a) HIPCHECK(this, track(this, &V.xb[a], n));
   // slots of the candidate list are stored in 27 bits
   if (nbr_cap > 0 && n > 0 && n < ((size_t)1 << dem::kNbrSlotBitsHost)) {
     HIPCHECK(this, track(this, &d_nbr_count, n));
@@ -131,6 +133,7 @@ int grains_gpu_system::allocate() {
   } else {
     d_nbr_count = nullptr;
     d_nbr = nullptr;
+    verlet = false;
   }
   pair_cap = (int)std::min<size_t>(16 * n + 4096, (size_t)1 << 28);
   HIPCHECK(this, track(this, &d_pairs, 2 * (size_t)pair_cap));
@@ -182,6 +185,38 @@ int grains_gpu_system::build_grid() {
   }
   if (total > (1L << 30)) return fail(this, GRAINS_CONFIG_ERROR, "too many linked cells");
   grid.ncells = (int)total;
+
+  // Verlet list grid: cells at least 2 Rmax factor + skin wide over the
+  // domain itself (no ghost layers: the list sees periodic neighbours at
+  // their nearest image), three or more per periodic axis so that no cell
+  // is a neighbour twice.  Spheres only.
+  verlet = false;
+  const double skin = verlet_skin < 0. ? 0.1 * cellsize : verlet_skin;
+  if (nbr_cap > 0 && skin > 0. && !has_shapes() && cellsize > kEps && cellsize < 1.e15) {
+    dem::Grid g{};
+    bool ok = true;
+    long tv = 1;
+    for (int a = 0; a < 3; ++a) {
+      double L = cfg.max[a] - cfg.origin[a];
+      int n = (int)((L + kEps) / (cellsize + skin));
+      if (n < 1) n = 1;
+      if (grid.periodic[a] && n < 3) ok = false;
+      g.edge[a] = L / n;
+      g.L[a] = L;
+      g.periodic[a] = grid.periodic[a];
+      g.n[a] = n;
+      g.lo[a] = cfg.origin[a];
+      g.off[a] = 0;
+      tv *= n;
+    }
+    g.ncells = (int)tv;
+    if (ok && tv <= grid.ncells) {
+      verlet = true;
+      V.Gv = g;
+      V.skin = skin;
+      V.half2 = 0.25 * skin * skin;
+    }
+  }
   return GRAINS_OK;
 }
 
@@ -257,7 +292,13 @@ StepParams grains_gpu_system::params() const {
 
 int grains_gpu_system::bin_current() {
   StepParams sp = params();
-  HIPCHECK(this, launch_key(P, S, grid, sp, d_status, stream));
+  if (verlet) {
+    // periodic wrap of the current positions, then a Verlet list build
+    HIPCHECK(this, launch_key(P, S, grid, sp, d_status, vstate(-1), stream));
+    HIPCHECK(this, launch_verlet_build(P, S, V, nparticles, cur & 1, d_status, nbr_list(), 0, stream));
+    return GRAINS_OK;
+  }
+  HIPCHECK(this, launch_key(P, S, grid, sp, d_status, vstate(0), stream));
   HIPCHECK(this, launch_bin(P, S, grid, nparticles, cur & 1, stream));
   return GRAINS_OK;
 }
@@ -275,10 +316,15 @@ int grains_gpu_system::mark(std::vector<hipEvent_t>& evs, size_t& n) {
   return GRAINS_OK;
 }
 
+// Binning after a drift; with a Verlet list, its rebuild, which runs only
+// if the drift (the launch with the current stamp) found the list stale.
 int grains_gpu_system::bin_launch() {
   int e;
   if (prof && (e = mark(prof->bin_ev, prof->nbin))) return e;
-  HIPCHECK(this, launch_bin(P, S, grid, nparticles, cur & 1, stream));
+  if (verlet)
+    HIPCHECK(this, launch_verlet_build(P, S, V, nparticles, cur & 1, d_status, nbr_list(), stamp, stream));
+  else
+    HIPCHECK(this, launch_bin(P, S, grid, nparticles, cur & 1, stream));
   if (prof && (e = mark(prof->bin_ev, prof->nbin))) return e;
   return GRAINS_OK;
 }
@@ -289,14 +335,14 @@ int grains_gpu_system::forces(int mode, int first_step) {
   StepParams sp = params();
   sp.first_step = first_step;
   const dem::NbrList L = nbr_list();
-  if (L.count) {
+  if (L.count && !verlet) {
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
     HIPCHECK(this, launch_detect(S, grid, nparticles, L, stream));
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
   }
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
   HIPCHECK(this, launch_forces(mode, cfg.precision == GRAINS_MIXED, P, S, grid, d_obs, d_models, sp, d_status,
-                               d_pairs, L, stream));
+                               d_pairs, L, vstate(next_stamp()), stream));
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
   if (mode == kModeForcesKick) cur ^= 2;
   if (mode == kModeFused) cur ^= 3;
@@ -328,7 +374,7 @@ int grains_gpu_system::enqueue_steps_impl(int nsteps) {
     // Move(dt/2, dt) -> LinkUpdate -> forces -> advanceVelocity(dt/2); the
     // forces, the second half kick and the next Move share one kernel.
     StepParams sp = params();
-    HIPCHECK(this, launch_move(P, S, grid, sp, d_status, 0.5 * cfg.dt, cfg.dt, stream));
+    HIPCHECK(this, launch_move(P, S, grid, sp, d_status, 0.5 * cfg.dt, cfg.dt, vstate(next_stamp()), stream));
     if ((e = bin_launch())) return e;
     if (recording && (e = record_state())) return e;
     for (int i = 1; i < nsteps; ++i) {
@@ -899,10 +945,20 @@ int grains_gpu_get_masses(grains_gpu_system* s, double* mass, double* inertia, d
   return GRAINS_OK;
 }
 
-int grains_gpu_set_neighbor_list(grains_gpu_system* s, int capacity) {
-  if (!s || capacity < 0) return GRAINS_ARGUMENT_ERROR;
+int grains_gpu_set_neighbor_list(grains_gpu_system* s, int capacity, double skin) {
+  if (!s || capacity < 0 || skin != skin) return GRAINS_ARGUMENT_ERROR;
   if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "the neighbour list is sized before initialisation");
   s->nbr_cap = capacity;
+  s->verlet_skin = skin;
+  return GRAINS_OK;
+}
+
+int grains_gpu_neighbor_stats(grains_gpu_system* s, double* skin, long* builds) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  int e = s->check_status();
+  if (e) return e;
+  if (skin) *skin = s->verlet ? s->V.skin : 0.;
+  if (builds) *builds = s->verlet ? s->last_status.n_rebuild : 0;
   return GRAINS_OK;
 }
 

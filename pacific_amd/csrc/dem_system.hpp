@@ -72,6 +72,20 @@ struct grains_gpu_system {
   int* d_nbr_count = nullptr;
   unsigned* d_nbr = nullptr;
   dem::NbrList nbr_list() const { return dem::NbrList{d_nbr_count, d_nbr, nbr_cap, nparticles}; }
+  // Verlet list: candidates within the contact distance plus a skin, binned
+  // on a grid of its own and rebuilt only once some particle has moved more
+  // than half the skin (requested skin < 0: a tenth of the linked-cell size;
+  // 0: the candidate list is rebuilt every step)
+  double verlet_skin = -1.;
+  bool verlet = false;  // decided at initialisation
+  dem::VerletState V{};
+  int stamp = 0;
+  int next_stamp() { return stamp = stamp >= (1 << 30) ? 1 : stamp + 1; }
+  dem::VerletState vstate(int st) const {
+    dem::VerletState v = V;
+    v.stamp = verlet ? st : 0;
+    return v;
+  }
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<void*> allocations;

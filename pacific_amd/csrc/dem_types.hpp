@@ -117,6 +117,22 @@ struct NbrList {
   int cap, n;
 };
 
+// Verlet neighbour list: every pair closer than the contact distance plus
+// a skin at the last build, kept until some particle has moved more than
+// half the skin since then (each contact of the meantime is then a listed
+// pair), so that binning and the neighbour search run only at rebuilds.
+// The list is binned on its own grid, whose cells (at least 2 Rmax factor +
+// skin wide) hold every such pair within 27 cells; periodic role swaps keep
+// following the linked-cell grid.
+struct VerletState {
+  Grid Gv;
+  double* xb[3];  // positions at the last build (storage order)
+  double skin;
+  double half2;   // (skin / 2)^2
+  int stamp;      // step stamp of this launch (> 0); 0 = no Verlet list
+  int pad;
+};
+
 // Device-side error / statistics block (one per system)
 struct DevStatus {
   int code;          // first error: 1 contact, 2 motion, 3 simulation
@@ -127,6 +143,8 @@ struct DevStatus {
   int n_recovered;   // ContactError recoveries
   int pad;
   double overlap_max;
+  int rebuild;       // step stamp of the last launch that found the Verlet list stale
+  int n_rebuild;     // Verlet list builds
 };
 
 enum ErrCode { kOk = 0, kContactError = 1, kMotionError = 2, kSimulationError = 3 };

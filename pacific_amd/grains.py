@@ -46,6 +46,7 @@ EXPORTS = [
     "grains_gpu_get_basis", "grains_gpu_set_basis", "grains_gpu_create_coupled", "grains_gpu_fluid_step",
     "grains_gpu_to_fluid", "grains_gpu_set_velocities", "grains_gpu_count_rigid_bodies",
     "grains_gpu_time_interval", "grains_gpu_closest_point", "grains_gpu_set_neighbor_list",
+    "grains_gpu_neighbor_stats",
 ]
 
 
@@ -76,7 +77,8 @@ def lib():
         L.grains_gpu_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
         L.grains_gpu_get_masses.argtypes = [vp, dp, dp, dp]
         L.grains_gpu_set_record_pairs.argtypes = [vp, ctypes.c_int]
-        L.grains_gpu_set_neighbor_list.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_set_neighbor_list.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.grains_gpu_neighbor_stats.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
         L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
         L.grains_gpu_num_particles.argtypes = [vp]
         L.grains_gpu_num_particles.restype = sz
@@ -383,10 +385,17 @@ class GrainsGPU:
         self._check(lib().grains_gpu_get_masses(self.h, _d(m), _d(I), _d(wgt)))
         return m, I, wgt
 
-    def neighbor_list(self, capacity):
-        """Candidates per particle of the per-step neighbour list (0: the force
-        sweep searches the 27 cells itself).  Before init() only."""
-        self._check(lib().grains_gpu_set_neighbor_list(self.h, int(capacity)))
+    def neighbor_list(self, capacity=32, skin=-1.0):
+        """Neighbour candidates per particle (0: the force sweep searches the
+        27 cells itself) and skin of the Verlet list (< 0: a tenth of the
+        cell size; 0: the list is rebuilt every step).  Before init() only."""
+        self._check(lib().grains_gpu_set_neighbor_list(self.h, int(capacity), float(skin)))
+
+    def neighbor_stats(self):
+        """(skin of the Verlet list in use or 0, number of list builds)"""
+        skin, builds = ctypes.c_double(), ctypes.c_long()
+        self._check(lib().grains_gpu_neighbor_stats(self.h, ctypes.byref(skin), ctypes.byref(builds)))
+        return skin.value, builds.value
 
     def record_pairs(self, on=True):
         self._check(lib().grains_gpu_set_record_pairs(self.h, 1 if on else 0))

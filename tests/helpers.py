@@ -14,13 +14,15 @@ def make_oracle(wl, scheme=0, fluid_density=0.0):
     return o
 
 
-def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None):
+def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None, skin=None):
+    """nbr: candidates per particle (0: cell search in the force sweep);
+    skin: Verlet skin (0: per-step list); None: the library defaults."""
     from pacific_amd import GrainsGPU
     g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], fluid_density,
                   wl["dt"], scheme, precision)
     workloads.build(g, wl)
-    if nbr is not None:
-        g.neighbor_list(nbr)
+    if nbr is not None or skin is not None:
+        g.neighbor_list(32 if nbr is None else nbr, -1.0 if skin is None else skin)
     if record:
         g.record_pairs(True)
     g.init()

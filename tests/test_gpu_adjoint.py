@@ -50,7 +50,8 @@ def J_of(st, A, B, C):
 
 
 def setup(wl):
-    g = make_gpu(wl, record=False)
+    # the adjoint replays the per-step neighbour search (no Verlet skin)
+    g = make_gpu(wl, record=False, skin=0.0)
     s0 = g.state()
     model = list(wl["models"][0][2:5])  # kn, en, muc
     return g, s0, model

@@ -1,15 +1,23 @@
-"""The per-step neighbour candidate list (k_detect, then the force sweep over
-the listed candidates) against the force sweep that searches the 27 cells
-itself: same candidates in the same order, so every state array is
-bit-identical after many steps, for spheres with walls, periodic gases,
-dense packings (many candidates per particle, and with a capacity so small
-that most particles overflow the list and are swept in full) and polyhedra.
+"""Neighbour candidate lists against the force sweep that searches the 27
+cells itself.
+
+* per-step list (k_detect after every binning, skin 0): same candidates in
+  the same order, so every state array is bit-identical after many steps,
+  for spheres with walls, periodic gases, dense packings (also with a
+  capacity so small that most particles overflow the list and are searched
+  in their cells) and polyhedra;
+* Verlet list (skin > 0, rebuilt only once a particle has moved more than
+  half the skin): at any step, the contacts found from the list -- built
+  several steps earlier -- are the contacts of a fresh cell search of the
+  same state, with the same per-contact arithmetic: equal contact records,
+  forces and torques within 1e-12 relative (the contact sums run in the
+  list's order), and the list is reused over many steps.
 """
 import numpy as np
 import pytest
 
 from pacific_amd import MIXED, workloads
-from tests.helpers import make_gpu, pair_set
+from tests.helpers import contact_scale, make_gpu, pair_set, rel_force_error
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +37,7 @@ def test_list_equals_cell_sweep(gpu, case, cap, precision):
     make, nsteps = CASES[case]
     wl = make()
     a = make_gpu(wl, precision=precision, nbr=0)
-    b = make_gpu(wl, precision=precision, nbr=cap)
+    b = make_gpu(wl, precision=precision, nbr=cap, skin=0.0)
     a.step(nsteps)
     b.step(nsteps)
     sa, sb = a.state(), b.state()
@@ -39,3 +47,63 @@ def test_list_equals_cell_sweep(gpu, case, cap, precision):
     assert pa == pb
     if case.startswith("dense"):
         assert len(pa) > 3 * a.n  # several candidates per particle: cap 3 overflows
+
+
+CASES["inclined"] = (lambda: workloads.inclined_plane(5000), 300)
+VERLET_CASES = ["inclined", "gas", "dense_periodic", "dense_walls"]
+
+
+@pytest.mark.parametrize("precision", [0, MIXED])
+@pytest.mark.parametrize("cap", [32, 3])
+@pytest.mark.parametrize("case", VERLET_CASES)
+def test_verlet_contacts_equal_cell_search(gpu, case, cap, precision):
+    make, nsteps = CASES[case]
+    wl = make()
+    v = make_gpu(wl, precision=precision, nbr=cap)  # default skin: Verlet list
+    c = make_gpu(wl, precision=precision, nbr=0)
+    skin, _ = v.neighbor_stats()
+    assert skin > 0
+    chunk = max(nsteps // 4, 1)
+    npairs = 0
+    for _ in range(4):
+        v.step(chunk)
+        sv = v.state()
+        # forces of the current state from the list as it stands (built at
+        # some earlier step) and from a fresh cell search of the same state
+        v.compute_forces()
+        fv = v.state()
+        c.set_state(x=sv["x"], v=sv["v"], w=sv["w"], q=sv["q"], recompute=True)
+        c.compute_forces()
+        fc = c.state()
+        pv, pc = pair_set(v.pairs()), pair_set(c.pairs())
+        assert pv == pc
+        npairs += len(pv)
+        if precision == 0:
+            _, _, wgt = c.masses()
+            scale = contact_scale(fc, wgt)
+            assert rel_force_error(fv["F"], fc["F"], scale) <= 1e-12
+            assert rel_force_error(fv["T"], fc["T"], scale * 1e-3) <= 1e-12
+        else:
+            # FP32 contact sums in another order: rounding of the largest
+            # contact forces, which may cancel in a particle's net force
+            for k in ("F", "T"):
+                assert np.abs(fv[k] - fc[k]).max() <= 1e-6 * np.abs(fc[k]).max(), k
+    _, builds = v.neighbor_stats()
+    assert 1 <= builds < 4 * chunk  # the list outlives single steps
+    assert npairs > 0
+
+
+def test_verlet_rebuilds_track_motion(gpu):
+    """A fast periodic gas rebuilds its list often, a packing at rest rarely;
+    either way the contact records match a fresh cell search step after
+    step."""
+    fast = make_gpu(workloads.periodic_gas(27000, sigma=0.6))
+    slow = make_gpu(workloads.dense_pack(side=8, sigma=1e-4, periodic=True, gravity=(0.0, 0.0, 0.0)))
+    ref = {}
+    for name, g in (("fast", fast), ("slow", slow)):
+        _, b0 = g.neighbor_stats()
+        g.step(100)
+        _, b1 = g.neighbor_stats()
+        ref[name] = b1 - b0
+    assert ref["fast"] > ref["slow"]
+    assert ref["fast"] >= 5
