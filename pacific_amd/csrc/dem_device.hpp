@@ -788,9 +788,8 @@ __device__ __forceinline__ void visit_candidate(const Particles& P, const Sorted
 // the domain -- every pair in contact -- that is the clone position x + vec
 // of the cell sweep, bit for bit.
 template <class V, bool SHAPES>
-__device__ __forceinline__ void visit_current(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
-                                              V& vis, const ConvexGeo* mg, int cx, int s2i) {
-  d3 xo{P.x[cx][0][s2i], P.x[cx][1][s2i], P.x[cx][2][s2i]};
+__device__ __forceinline__ void visit_current_at(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
+                                                 V& vis, const ConvexGeo* mg, int cx, int s2i, d3 xo, double ro) {
   double* xs[3] = {&xo.x, &xo.y, &xo.z};
   const double* ms[3] = {&me.x.x, &me.x.y, &me.x.z};
 #pragma unroll
@@ -800,12 +799,37 @@ __device__ __forceinline__ void visit_current(const Particles& P, const Grid& G,
       if (d > 0.5 * G.L[a]) *xs[a] = *xs[a] + image_shift(G, a, -1);
       else if (d < -0.5 * G.L[a]) *xs[a] = *xs[a] + image_shift(G, a, 1);
     }
-  const double ro = P.rad[s2i];
   d3 AB = xo - me.x;
   double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
   double R = me.r + ro;
   if (s2 > R * R * 1.000000000001) return;
   narrow_visit<V, SHAPES>(P, st, me, vis, mg, cx, -1, s2i, P.id[s2i], xo, ro, AB, sqrt(s2));
+}
+
+template <class V, bool SHAPES>
+__device__ __forceinline__ void visit_current(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
+                                              V& vis, const ConvexGeo* mg, int cx, int s2i) {
+  visit_current_at<V, SHAPES>(P, G, st, me, vis, mg, cx, s2i, d3{P.x[cx][0][s2i], P.x[cx][1][s2i], P.x[cx][2][s2i]},
+                              P.rad[s2i]);
+}
+
+// The first Verlet candidate of a particle, requested together with the
+// particle's own state so that the list's dependent loads overlap it.
+struct VerletHint {
+  int c;        // candidates listed
+  unsigned e0;  // first entry
+  d3 x0;        // its position and radius (meaningful when c >= 1)
+  double r0;
+};
+
+__device__ __forceinline__ VerletHint verlet_hint(const Particles& P, const NbrList& L, int n, int cx, int s) {
+  VerletHint h;
+  h.c = L.count[s];
+  h.e0 = L.ent[s];  // entry 0 of particle s: always inside the list
+  const int p0 = h.e0 < (unsigned)n ? (int)h.e0 : s;
+  h.x0 = d3{P.x[cx][0][p0], P.x[cx][1][p0], P.x[cx][2][p0]};
+  h.r0 = P.rad[p0];
+  return h;
 }
 
 // Particle-obstacle contacts (SimpleObstacle::InterAction,
@@ -912,10 +936,16 @@ template <class V, bool SHAPES = false>
 __device__ __forceinline__ void for_each_verlet(const Particles& P, const Sorted& S, const Grid& G,
                                                 const VerletState& VS, const BoxObstacle* __restrict__ obs, int nobs,
                                                 DevStatus* st, int n, int s, const Self& me, V& vis,
-                                                const NbrList& L, const ConvexGeo* mg = nullptr, int cx = 0) {
-  const int c = L.count[s];
+                                                const NbrList& L, const ConvexGeo* mg = nullptr, int cx = 0,
+                                                const VerletHint* h = nullptr) {
+  const int c = h ? h->c : L.count[s];
   if (c <= L.cap) {
-    for (int i = 0; i < c; ++i) {
+    int i = 0;
+    if (h && c > 0) {
+      if (h->e0 < (unsigned)n) visit_current_at<V, SHAPES>(P, G, st, me, vis, mg, cx, (int)h->e0, h->x0, h->r0);
+      i = 1;
+    }
+    for (; i < c; ++i) {
       const unsigned s2i = L.ent[(size_t)i * L.n + s];
       if (s2i < (unsigned)n) visit_current<V, SHAPES>(P, G, st, me, vis, mg, cx, (int)s2i);
     }
@@ -1006,10 +1036,10 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
                                                int cv, int k, const Self& me, d3& F, d3& T,
                                                const ConvexGeo* mg = nullptr, int cx = 0,
                                                const NbrList* L = nullptr, const VerletState* VS = nullptr,
-                                               int s = -1) {
+                                               int s = -1, const VerletHint* h = nullptr) {
   ForceVisitor<MIXED> vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
   if (VS && VS->stamp)
-    for_each_verlet<ForceVisitor<MIXED>, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx);
+    for_each_verlet<ForceVisitor<MIXED>, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx, h);
   else if (L && L->count)
     for_each_listed<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx);
   else

@@ -22,6 +22,7 @@
 // built with -ffp-contract=off; the mixed kernels evaluate the force model in
 // FP32 (contact decisions stay FP64 so the contact list is identical).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 
 #include "dem_device.hpp"
@@ -178,10 +179,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int* __restric
 // Storage index -> (cell start + arrival slot)
 __global__ __launch_bounds__(256) void k_scatter(Sorted S, int n, const int* gate, int stamp) {
   if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int dst = S.start[S.key[s]] + S.slot[s];
-  if ((unsigned)dst < (unsigned)n) S.tmp_order[dst] = s;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    int dst = S.start[S.key[s]] + S.slot[s];
+    if ((unsigned)dst < (unsigned)n) S.tmp_order[dst] = s;
+  }
 }
 
 // One thread per cell: order the cell's particles by id (deterministic, and
@@ -190,10 +191,10 @@ __global__ __launch_bounds__(256) void k_scatter(Sorted S, int n, const int* gat
 __global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells, int cx, int n, const int* gate,
                                                int stamp) {
   if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  S.count[c] = 0;
-  canon_cell(P, S, min(S.start[c], n), min(S.start[c + 1], n), cx, 0, n);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+    S.count[c] = 0;
+    canon_cell(P, S, min(S.start[c], n), min(S.start[c + 1], n), cx, 0, n);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -250,7 +251,9 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   // only after an interrupted binning
   if ((unsigned)s >= (unsigned)sp.n || (unsigned)k >= (unsigned)sp.n) return;
   Self me;
+  VerletHint hint{};
   if (VS.stamp) {
+    hint = verlet_hint(P, L, sp.n, cx, s);
     // Verlet list: the sorted arrays hold the positions of the last build
     me.x = ld3(P.x[cx], s);
     me.r = P.rad[s];
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   ConvexGeo mg;
   if (SHAPES) mg = convex_of(P, s, me.x, cx);
   sweep_particle<MIXED, SHAPES>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
-                                cx, &L, &VS, s);
+                                cx, &L, &VS, s, VS.stamp ? &hint : nullptr);
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
   if (!(MODE == kFused && sp.scheme == kLeapFrog)) {
@@ -379,6 +382,9 @@ hipError_t launch_key(const Particles& P, const Sorted& S, const Grid& G, const 
   return hipGetLastError();
 }
 
+// Blocks of a gated launch (grid-stride loops): enough to fill the chip.
+constexpr int kGatedBlocks = 2048;
+
 hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n, int cx, hipStream_t stream,
                       const int* gate, int stamp) {
   int nb = nblk(G.ncells, kScanTile);
@@ -387,8 +393,11 @@ hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n,
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, S.block_sums, nb, gate, stamp);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums,
                      S.start, n, gate, stamp);
-  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, stream, S, n, gate, stamp);
-  hipLaunchKernelGGL(k_canon, dim3(nblk(G.ncells, 256)), dim3(256), 0, stream, P, S, G.ncells, cx, n, gate, stamp);
+  // gated launches mostly find nothing to do: small grids that stride
+  const int cap = gate ? kGatedBlocks : (1 << 30);
+  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(std::min(nblk(n, 256), cap)), dim3(256), 0, stream, S, n, gate, stamp);
+  hipLaunchKernelGGL(k_canon, dim3(std::min(nblk(G.ncells, 256), cap)), dim3(256), 0, stream, P, S, G.ncells, cx, n,
+                     gate, stamp);
   return hipGetLastError();
 }
 
@@ -399,56 +408,59 @@ __global__ __launch_bounds__(256) void k_vkey(Particles P, Sorted S, VerletState
                                               int stamp) {
   const int* gate = stamp ? &st->rebuild : nullptr;
   if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) atomicAdd(&st->n_rebuild, 1);
-  if (s >= n) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&st->n_rebuild, 1);
   const Grid& G = V.Gv;
-  const double x[3] = {P.x[cx][0][s], P.x[cx][1][s], P.x[cx][2][s]};
-  int c[3];
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const double x[3] = {P.x[cx][0][s], P.x[cx][1][s], P.x[cx][2][s]};
+    int c[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    c[a] = min(max(ref_cell(x[a], G, a), 0), G.n[a] - 1);
-    V.xb[a][s] = x[a];
+    for (int a = 0; a < 3; ++a) {
+      c[a] = min(max(ref_cell(x[a], G, a), 0), G.n[a] - 1);
+      V.xb[a][s] = x[a];
+    }
+    const int key = (c[1] * G.n[2] + c[2]) * G.n[0] + c[0];
+    S.key[s] = key;
+    S.slot[s] = atomicAdd(&S.count[key], 1);
   }
-  const int key = (c[1] * G.n[2] + c[2]) * G.n[0] + c[0];
-  S.key[s] = key;
-  S.slot[s] = atomicAdd(&S.count[key], 1);
 }
 
 __global__ __launch_bounds__(256) void k_vdetect(Sorted S, VerletState V, int n, NbrList L, const int* gate,
                                                  int stamp) {
   if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
-  const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int s = S.order[k];
-  if ((unsigned)s >= (unsigned)n) return;
-  const double* pr = S.posr + 4 * (size_t)k;
-  const d3 x{pr[0], pr[1], pr[2]};
-  const double r = pr[3] + V.skin;
-  int c = 0;
-  for_each_candidate(S, V.Gv, 0, 0, n, x, [&](int k2, int ix, int iy, int iz) {
-    if (k2 == k) return;
-    const double* q2 = S.posr + 4 * (size_t)k2;
-    d3 xo{ix ? q2[0] + image_shift(V.Gv, 0, ix) : q2[0], iy ? q2[1] + image_shift(V.Gv, 1, iy) : q2[1],
-          iz ? q2[2] + image_shift(V.Gv, 2, iz) : q2[2]};
-    d3 AB = xo - x;
-    double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
-    double R = r + q2[3];
-    if (s2 > R * R * 1.000000000001) return;
-    if (c < L.cap) L.ent[(size_t)c * n + s] = (unsigned)S.order[k2];
-    ++c;
-  });
-  L.count[s] = c;
+  for (int blk = xcd_tile(blockIdx.x, gridDim.x); blk * 256 < n; blk += gridDim.x) {
+    const int k = blk * 256 + threadIdx.x;
+    if (k >= n) break;
+    const int s = S.order[k];
+    if ((unsigned)s >= (unsigned)n) continue;
+    const double* pr = S.posr + 4 * (size_t)k;
+    const d3 x{pr[0], pr[1], pr[2]};
+    const double r = pr[3] + V.skin;
+    int c = 0;
+    for_each_candidate(S, V.Gv, 0, 0, n, x, [&](int k2, int ix, int iy, int iz) {
+      if (k2 == k) return;
+      const double* q2 = S.posr + 4 * (size_t)k2;
+      d3 xo{ix ? q2[0] + image_shift(V.Gv, 0, ix) : q2[0], iy ? q2[1] + image_shift(V.Gv, 1, iy) : q2[1],
+            iz ? q2[2] + image_shift(V.Gv, 2, iz) : q2[2]};
+      d3 AB = xo - x;
+      double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
+      double R = r + q2[3];
+      if (s2 > R * R * 1.000000000001) return;
+      if (c < L.cap) L.ent[(size_t)c * n + s] = (unsigned)S.order[k2];
+      ++c;
+    });
+    L.count[s] = c;
+  }
 }
 
 hipError_t launch_verlet_build(const Particles& P, const Sorted& S, const VerletState& V, int n, int cx,
                                DevStatus* st, const NbrList& L, int stamp, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vkey, dim3(nblk(n, 256)), dim3(256), 0, stream, P, S, V, n, cx, st, stamp);
+  const int nb = stamp ? std::min(nblk(n, 256), kGatedBlocks) : nblk(n, 256);
+  hipLaunchKernelGGL(k_vkey, dim3(nb), dim3(256), 0, stream, P, S, V, n, cx, st, stamp);
   const int* gate = stamp ? &st->rebuild : nullptr;
   hipError_t e = launch_bin(P, S, V.Gv, n, cx, stream, gate, stamp);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_vdetect, dim3(nblk(n, 256)), dim3(256), 0, stream, S, V, n, L, gate, stamp);
+  hipLaunchKernelGGL(k_vdetect, dim3(nb), dim3(256), 0, stream, S, V, n, L, gate, stamp);
   return hipGetLastError();
 }
 
