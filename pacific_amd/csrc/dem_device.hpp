@@ -787,11 +787,9 @@ __device__ __forceinline__ void visit_candidate(const Particles& P, const Sorted
 // periodic image nearest to this particle: for any pair closer than half
 // the domain -- every pair in contact -- that is the clone position x + vec
 // of the cell sweep, bit for bit.
-template <class V, bool SHAPES>
-__device__ __forceinline__ void visit_current_at(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
-                                                 V& vis, const ConvexGeo* mg, int cx, int s2i, d3 xo, double ro) {
+__device__ __forceinline__ d3 nearest_image(const Grid& G, d3 xo, const d3& x) {
   double* xs[3] = {&xo.x, &xo.y, &xo.z};
-  const double* ms[3] = {&me.x.x, &me.x.y, &me.x.z};
+  const double* ms[3] = {&x.x, &x.y, &x.z};
 #pragma unroll
   for (int a = 0; a < 3; ++a)
     if (G.periodic[a]) {
@@ -799,6 +797,13 @@ __device__ __forceinline__ void visit_current_at(const Particles& P, const Grid&
       if (d > 0.5 * G.L[a]) *xs[a] = *xs[a] + image_shift(G, a, -1);
       else if (d < -0.5 * G.L[a]) *xs[a] = *xs[a] + image_shift(G, a, 1);
     }
+  return xo;
+}
+
+template <class V, bool SHAPES>
+__device__ __forceinline__ void visit_current_at(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
+                                                 V& vis, const ConvexGeo* mg, int cx, int s2i, d3 xo, double ro) {
+  xo = nearest_image(G, xo, me.x);
   d3 AB = xo - me.x;
   double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
   double R = me.r + ro;
@@ -811,25 +816,6 @@ __device__ __forceinline__ void visit_current(const Particles& P, const Grid& G,
                                               V& vis, const ConvexGeo* mg, int cx, int s2i) {
   visit_current_at<V, SHAPES>(P, G, st, me, vis, mg, cx, s2i, d3{P.x[cx][0][s2i], P.x[cx][1][s2i], P.x[cx][2][s2i]},
                               P.rad[s2i]);
-}
-
-// The first Verlet candidate of a particle, requested together with the
-// particle's own state so that the list's dependent loads overlap it.
-struct VerletHint {
-  int c;        // candidates listed
-  unsigned e0;  // first entry
-  d3 x0;        // its position and radius (meaningful when c >= 1)
-  double r0;
-};
-
-__device__ __forceinline__ VerletHint verlet_hint(const Particles& P, const NbrList& L, int n, int cx, int s) {
-  VerletHint h;
-  h.c = L.count[s];
-  h.e0 = L.ent[s];  // entry 0 of particle s: always inside the list
-  const int p0 = h.e0 < (unsigned)n ? (int)h.e0 : s;
-  h.x0 = d3{P.x[cx][0][p0], P.x[cx][1][p0], P.x[cx][2][p0]};
-  h.r0 = P.rad[p0];
-  return h;
 }
 
 // Particle-obstacle contacts (SimpleObstacle::InterAction,
@@ -937,15 +923,20 @@ __device__ __forceinline__ void for_each_verlet(const Particles& P, const Sorted
                                                 const VerletState& VS, const BoxObstacle* __restrict__ obs, int nobs,
                                                 DevStatus* st, int n, int s, const Self& me, V& vis,
                                                 const NbrList& L, const ConvexGeo* mg = nullptr, int cx = 0,
-                                                const VerletHint* h = nullptr) {
-  const int c = h ? h->c : L.count[s];
-  if (c <= L.cap) {
-    int i = 0;
-    if (h && c > 0) {
-      if (h->e0 < (unsigned)n) visit_current_at<V, SHAPES>(P, G, st, me, vis, mg, cx, (int)h->e0, h->x0, h->r0);
-      i = 1;
+                                                const NbrList* CL = nullptr) {
+  // this step's contacts, filtered from the list (k_vfilter) in its order
+  const int c2 = CL ? CL->count[s] : 0;
+  if (CL && c2 <= CL->cap) {
+    for (int i = 0; i < c2; ++i) {
+      const unsigned s2i = CL->ent[(size_t)i * CL->n + s];
+      if (s2i < (unsigned)n) visit_current<V, SHAPES>(P, G, st, me, vis, mg, cx, (int)s2i);
     }
-    for (; i < c; ++i) {
+    visit_obstacles<V, SHAPES>(obs, nobs, st, me, vis, mg);
+    return;
+  }
+  const int c = L.count[s];
+  if (c <= L.cap) {
+    for (int i = 0; i < c; ++i) {
       const unsigned s2i = L.ent[(size_t)i * L.n + s];
       if (s2i < (unsigned)n) visit_current<V, SHAPES>(P, G, st, me, vis, mg, cx, (int)s2i);
     }
@@ -1036,10 +1027,11 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
                                                int cv, int k, const Self& me, d3& F, d3& T,
                                                const ConvexGeo* mg = nullptr, int cx = 0,
                                                const NbrList* L = nullptr, const VerletState* VS = nullptr,
-                                               int s = -1, const VerletHint* h = nullptr) {
+                                               int s = -1, const NbrList* CL = nullptr) {
   ForceVisitor<MIXED> vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
   if (VS && VS->stamp)
-    for_each_verlet<ForceVisitor<MIXED>, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx, h);
+    for_each_verlet<ForceVisitor<MIXED>, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx,
+                                                 CL && CL->count ? CL : nullptr);
   else if (L && L->count)
     for_each_listed<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx);
   else

@@ -235,10 +235,58 @@ __global__ __launch_bounds__(256) void k_detect(Sorted S, Grid G, int n, NbrList
 // (the other schemes).
 enum { kForcesOnly = 0, kForcesKick = 1, kFused = 2 };
 
+// Second half kick of a sphere from its torsor (F, T), and inside a fused
+// step the next Move, periodic swap and cell key (or Verlet staleness check).
+template <int MODE>
+__device__ __forceinline__ void sphere_tail(const Particles& P, const Sorted& S, const Grid& G, const StepParams& sp,
+                                            DevStatus* st, const VerletState& VS, int s, int cx, int cv, Kin& kin,
+                                            double mass, double crust, int id, double cf, double I, const d3& F,
+                                            const d3& T) {
+  const int nv = cv ^ 1;
+  if (MODE == kForcesKick) {
+    lf_kick(kin, F, T, mass, cf, I, 0.5 * sp.dt);
+    st3(P.v[nv], s, kin.v);
+    st3(P.w[nv], s, kin.w);
+    st3(P.wb, s, kin.wb);
+    return;
+  }
+  // kFused
+  double depl;
+  if (sp.scheme == kLeapFrog) {
+    lf_kick(kin, F, T, mass, cf, I, 0.5 * sp.dt);
+    depl = lf_move(kin, F, T, mass, cf, I, 0.5 * sp.dt, sp.dt);
+  } else {
+    AB2State ab;
+    if (sp.scheme == kAdamsBashforth2) {
+      ab.v_nm2 = ld3(P.v_nm2, s);
+      ab.dUdt_nm2 = ld3(P.dUdt_nm2, s);
+      ab.wb_nm2 = ld3(P.wb_nm2, s);
+      ab.dWdt_nm2 = ld3(P.dWdt_nm2, s);
+    }
+    depl = other_move(sp.scheme, sp.first_step != 0, kin, ab, F, T, mass, cf, I, sp.dt, sp.dt);
+    if (sp.scheme == kAdamsBashforth2) {
+      st3(P.v_nm2, s, ab.v_nm2);
+      st3(P.dUdt_nm2, s, ab.dUdt_nm2);
+      st3(P.wb_nm2, s, ab.wb_nm2);
+      st3(P.dWdt_nm2, s, ab.dWdt_nm2);
+    }
+  }
+  if (depl > crust) raise_error(st, kMotionError, id, 0);
+  wrap_and_bin(kin, G, S, st, s, id, 0, &VS);
+  st3(P.x[cx ^ 1], s, kin.x);
+  st3(P.v[nv], s, kin.v);
+  st3(P.w[nv], s, kin.w);
+  st3(P.wb, s, kin.wb);
+  P.q[0][s] = kin.q.v.x;
+  P.q[1][s] = kin.q.v.y;
+  P.q[2][s] = kin.q.v.z;
+  P.q[3][s] = kin.q.w;
+}
+
 template <int MODE, bool MIXED, bool SHAPES>
 __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
                                                 const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
-                                                int* pairs, NbrList L, VerletState VS) {
+                                                int* pairs, NbrList L, VerletState VS, NbrList CL) {
   // Spheres: one thread per storage index, so that the particle's own state
   // is read and written in contiguous runs; its sorted slot locates the cell
   // data.  Non-spherical bodies: one thread per sorted slot, so that a wave
@@ -251,9 +299,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   // only after an interrupted binning
   if ((unsigned)s >= (unsigned)sp.n || (unsigned)k >= (unsigned)sp.n) return;
   Self me;
-  VerletHint hint{};
   if (VS.stamp) {
-    hint = verlet_hint(P, L, sp.n, cx, s);
     // Verlet list: the sorted arrays hold the positions of the last build
     me.x = ld3(P.x[cx], s);
     me.r = P.rad[s];
@@ -277,7 +323,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   ConvexGeo mg;
   if (SHAPES) mg = convex_of(P, s, me.x, cx);
   sweep_particle<MIXED, SHAPES>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
-                                cx, &L, &VS, s, VS.stamp ? &hint : nullptr);
+                                cx, &L, &VS, s, &CL);
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
   if (!(MODE == kFused && sp.scheme == kLeapFrog)) {
@@ -314,44 +360,45 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
     st3(P.wb, s, kin.wb);
     return;
   }
-  if (MODE == kForcesKick) {
-    lf_kick(kin, F, T, me.m, cf, I, 0.5 * sp.dt);
-    st3(P.v[nv], s, kin.v);
-    st3(P.w[nv], s, kin.w);
-    st3(P.wb, s, kin.wb);
-    return;
-  }
-  // kFused
-  double depl;
-  if (sp.scheme == kLeapFrog) {
-    lf_kick(kin, F, T, me.m, cf, I, 0.5 * sp.dt);
-    depl = lf_move(kin, F, T, me.m, cf, I, 0.5 * sp.dt, sp.dt);
+  sphere_tail<MODE>(P, S, G, sp, st, VS, s, cx, cv, kin, me.m, me.c, me.id, cf, I, F, T);
+}
+
+// ---------------------------------------------------------------------------
+// Contacts of a step with the Verlet list: the listed candidates of each
+// sphere that pass the narrow phase's conservative distance reject at the
+// current positions, in the list's order.  Few registers, so that many
+// waves overlap the dependent loads of the list; the force sweep then
+// visits only these pairs (same pairs, same order, same arithmetic).
+__global__ __launch_bounds__(256) void k_vfilter(Particles P, Sorted S, Grid G, VerletState VS, int n, int cx,
+                                                 NbrList L, NbrList CL) {
+  const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const d3 x = ld3(P.x[cx], s);
+  const double r = P.rad[s];
+  int c2 = 0;
+  auto test = [&](int s2i) {
+    const d3 xo = nearest_image(G, d3{P.x[cx][0][s2i], P.x[cx][1][s2i], P.x[cx][2][s2i]}, x);
+    const d3 AB = xo - x;
+    const double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
+    const double R = r + P.rad[s2i];
+    if (s2 > R * R * 1.000000000001) return;
+    if (c2 < CL.cap) CL.ent[(size_t)c2 * n + s] = (unsigned)s2i;
+    ++c2;
+  };
+  const int c = L.count[s];
+  if (c <= L.cap) {
+    for (int i = 0; i < c; ++i) {
+      const unsigned s2i = L.ent[(size_t)i * L.n + s];
+      if (s2i < (unsigned)n) test((int)s2i);
+    }
   } else {
-    AB2State ab;
-    if (sp.scheme == kAdamsBashforth2) {
-      ab.v_nm2 = ld3(P.v_nm2, s);
-      ab.dUdt_nm2 = ld3(P.dUdt_nm2, s);
-      ab.wb_nm2 = ld3(P.wb_nm2, s);
-      ab.dWdt_nm2 = ld3(P.dWdt_nm2, s);
-    }
-    depl = other_move(sp.scheme, sp.first_step != 0, kin, ab, F, T, me.m, cf, I, sp.dt, sp.dt);
-    if (sp.scheme == kAdamsBashforth2) {
-      st3(P.v_nm2, s, ab.v_nm2);
-      st3(P.dUdt_nm2, s, ab.dUdt_nm2);
-      st3(P.wb_nm2, s, ab.wb_nm2);
-      st3(P.dWdt_nm2, s, ab.dWdt_nm2);
-    }
+    const d3 xb{VS.xb[0][s], VS.xb[1][s], VS.xb[2][s]};
+    for_each_candidate(S, VS.Gv, 0, 0, n, xb, [&](int k2, int, int, int) {
+      const int s2i = S.order[k2];
+      if (s2i != s && (unsigned)s2i < (unsigned)n) test(s2i);
+    });
   }
-  if (depl > me.c) raise_error(st, kMotionError, me.id, 0);
-  wrap_and_bin(kin, G, S, st, s, me.id, 0, &VS);
-  st3(P.x[cx ^ 1], s, kin.x);
-  st3(P.v[nv], s, kin.v);
-  st3(P.w[nv], s, kin.w);
-  st3(P.wb, s, kin.wb);
-  P.q[0][s] = kin.q.v.x;
-  P.q[1][s] = kin.q.v.y;
-  P.q[2][s] = kin.q.v.z;
-  P.q[3][s] = kin.q.w;
+  CL.count[s] = c2;
 }
 
 // ---------------------------------------------------------------------------
@@ -469,13 +516,20 @@ int scan_blocks(int ncells) { return nblk(ncells, kScanTile); }
 template <int MODE, bool MIXED>
 static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
                             const HookeModel* models, const StepParams& sp, DevStatus* st, int* pairs,
-                            const NbrList& L, const VerletState& V, hipStream_t stream) {
+                            const NbrList& L, const VerletState& V, const NbrList& CL, hipStream_t stream) {
   if (P.shape)
     hipLaunchKernelGGL((k_forces<MODE, MIXED, true>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs, L, V);
+                       models, sp, st, pairs, L, V, CL);
   else
     hipLaunchKernelGGL((k_forces<MODE, MIXED, false>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs, L, V);
+                       models, sp, st, pairs, L, V, CL);
+}
+
+hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, const VerletState& V, int n, int cx,
+                          const NbrList& L, const NbrList& CL, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vfilter, dim3(nblk(n, 256)), dim3(256), 0, stream, P, S, G, V, n, cx, L, CL);
+  return hipGetLastError();
 }
 
 hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream) {
@@ -486,16 +540,17 @@ hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L
 
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
                          const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,
-                         int* pairs, const NbrList& L, const VerletState& V, hipStream_t stream) {
+                         int* pairs, const NbrList& L, const VerletState& V, const NbrList& CL,
+                         hipStream_t stream) {
   if (sp.n == 0) return hipSuccess;
   if (mixed) {
-    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, true>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
-    else if (mode == kForcesKick) launch_forces_t<kForcesKick, true>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
-    else launch_forces_t<kFused, true>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
+    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, true>(P, S, G, obs, models, sp, st, pairs, L, V, CL, stream);
+    else if (mode == kForcesKick) launch_forces_t<kForcesKick, true>(P, S, G, obs, models, sp, st, pairs, L, V, CL, stream);
+    else launch_forces_t<kFused, true>(P, S, G, obs, models, sp, st, pairs, L, V, CL, stream);
   } else {
-    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, false>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
-    else if (mode == kForcesKick) launch_forces_t<kForcesKick, false>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
-    else launch_forces_t<kFused, false>(P, S, G, obs, models, sp, st, pairs, L, V, stream);
+    if (mode == kForcesOnly) launch_forces_t<kForcesOnly, false>(P, S, G, obs, models, sp, st, pairs, L, V, CL, stream);
+    else if (mode == kForcesKick) launch_forces_t<kForcesKick, false>(P, S, G, obs, models, sp, st, pairs, L, V, CL, stream);
+    else launch_forces_t<kFused, false>(P, S, G, obs, models, sp, st, pairs, L, V, CL, stream);
   }
   return hipGetLastError();
 }

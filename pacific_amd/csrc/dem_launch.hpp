@@ -26,11 +26,15 @@ hipError_t launch_verlet_build(const Particles& P, const Sorted& S, const Verlet
 int scan_blocks(int ncells);
 // zero n ints with a kernel (safe inside captured graphs, unlike hipMemsetAsync)
 hipError_t launch_clear(int* p, int n, hipStream_t stream);
+// contacts of this step from the Verlet list L into CL
+hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, const VerletState& V, int n, int cx,
+                          const NbrList& L, const NbrList& CL, hipStream_t stream);
 // neighbour candidates of every sorted particle into L
 hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream);
 // force sweep over the candidates of L (L.count == nullptr: 27-cell sweep)
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
                          const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,
-                         int* pairs, const NbrList& L, const VerletState& V, hipStream_t stream);
+                         int* pairs, const NbrList& L, const VerletState& V, const NbrList& CL,
+                         hipStream_t stream);
 
 }  // namespace dem

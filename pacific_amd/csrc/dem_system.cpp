@@ -126,6 +126,10 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, track(this, &d_status, 1));
   if (verlet)
     for (int a = 0; a < 3; ++a) HIPCHECK(this, track(this, &V.xb[a], n));
+  if (verlet) {
+    HIPCHECK(this, track(this, &d_ct_count, n));
+    HIPCHECK(this, track(this, &d_ct, (size_t)kContactCap * n));
+  }
   // slots of the candidate list are stored in 27 bits
   if (nbr_cap > 0 && n > 0 && n < ((size_t)1 << dem::kNbrSlotBitsHost)) {
     HIPCHECK(this, track(this, &d_nbr_count, n));
@@ -340,9 +344,16 @@ int grains_gpu_system::forces(int mode, int first_step) {
     HIPCHECK(this, launch_detect(S, grid, nparticles, L, stream));
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
   }
+  const dem::NbrList CL = contact_list();
+  if (CL.count) {
+    // this step's contacts from the Verlet list
+    if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
+    HIPCHECK(this, launch_vfilter(P, S, grid, V, nparticles, cur & 1, L, CL, stream));
+    if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
+  }
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
   HIPCHECK(this, launch_forces(mode, cfg.precision == GRAINS_MIXED, P, S, grid, d_obs, d_models, sp, d_status,
-                               d_pairs, L, vstate(next_stamp()), stream));
+                               d_pairs, L, vstate(next_stamp()), CL, stream));
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
   if (mode == kModeForcesKick) cur ^= 2;
   if (mode == kModeFused) cur ^= 3;

@@ -79,6 +79,13 @@ struct grains_gpu_system {
   double verlet_skin = -1.;
   bool verlet = false;  // decided at initialisation
   dem::VerletState V{};
+  // contacts of the current step filtered from the Verlet list
+  static constexpr int kContactCap = 16;
+  int* d_ct_count = nullptr;
+  unsigned* d_ct = nullptr;
+  dem::NbrList contact_list() const {
+    return verlet ? dem::NbrList{d_ct_count, d_ct, kContactCap, nparticles} : dem::NbrList{};
+  }
   int stamp = 0;
   int next_stamp() { return stamp = stamp >= (1 << 30) ? 1 : stamp + 1; }
   dem::VerletState vstate(int st) const {

@@ -7,7 +7,10 @@ cells itself.
   capacity so small that most particles overflow the list and are searched
   in their cells) and polyhedra;
 * Verlet list (skin > 0, rebuilt only once a particle has moved more than
-  half the skin): at any step, the contacts found from the list -- built
+  half the skin; a step runs as contact sums, then integration): with
+  velocity-independent contact forces (en = 1, no friction) the torsor a
+  step leaves is that of one force launch over the same list, bit for bit;
+  at any step, the contacts found from the list -- built
   several steps earlier -- are the contacts of a fresh cell search of the
   same state, with the same per-contact arithmetic: equal contact records,
   forces and torques within 1e-12 relative (the contact sums run in the
@@ -107,3 +110,23 @@ def test_verlet_rebuilds_track_motion(gpu):
         ref[name] = b1 - b0
     assert ref["fast"] > ref["slow"]
     assert ref["fast"] >= 5
+
+
+@pytest.mark.parametrize("precision", [0, MIXED])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_split_step_torsor_equals_force_launch(gpu, precision, periodic):
+    """The torsor a step leaves (contact sums launch, then the integration
+    launch that ends the step) against one force launch at the same state;
+    elastic frictionless contacts, so that the force does not depend on the
+    half-step velocities the step used."""
+    wl = workloads.dense_pack(side=8, en=1.0, muc=0.0, periodic=periodic,
+                              gravity=(0.0, 0.0, 0.0) if periodic else (0.0, 0.0, -9.81))
+    g = make_gpu(wl, precision=precision)
+    assert g.neighbor_stats()[0] > 0
+    for _ in range(3):
+        g.step(40)
+        s = g.state()
+        g.compute_forces()
+        f = g.state()
+        assert np.array_equal(s["F"], f["F"]) and np.array_equal(s["T"], f["T"])
+    assert np.abs(s["F"]).max() > 0
