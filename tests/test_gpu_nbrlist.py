@@ -7,10 +7,11 @@ cells itself.
   capacity so small that most particles overflow the list and are searched
   in their cells) and polyhedra;
 * Verlet list (skin > 0, rebuilt only once a particle has moved more than
-  half the skin; a step runs as contact sums, then integration): with
-  velocity-independent contact forces (en = 1, no friction) the torsor a
-  step leaves is that of one force launch over the same list, bit for bit;
-  at any step, the contacts found from the list -- built
+  half the skin; each step filters the listed pairs down to its contacts,
+  then the force sweep visits those): with velocity-independent contact
+  forces (en = 1, no friction) the torsor a step leaves is that of a force
+  evaluation at the same state, bit for bit; at any step, the contacts
+  found from the list -- built
   several steps earlier -- are the contacts of a fresh cell search of the
   same state, with the same per-contact arithmetic: equal contact records,
   forces and torques within 1e-12 relative (the contact sums run in the
@@ -114,11 +115,10 @@ def test_verlet_rebuilds_track_motion(gpu):
 
 @pytest.mark.parametrize("precision", [0, MIXED])
 @pytest.mark.parametrize("periodic", [False, True])
-def test_split_step_torsor_equals_force_launch(gpu, precision, periodic):
-    """The torsor a step leaves (contact sums launch, then the integration
-    launch that ends the step) against one force launch at the same state;
-    elastic frictionless contacts, so that the force does not depend on the
-    half-step velocities the step used."""
+def test_step_torsor_equals_force_evaluation(gpu, precision, periodic):
+    """The torsor the launch that ends a step leaves against a separate force
+    evaluation at the same state; elastic frictionless contacts, so that the
+    force does not depend on the half-step velocities the step used."""
     wl = workloads.dense_pack(side=8, en=1.0, muc=0.0, periodic=periodic,
                               gravity=(0.0, 0.0, 0.0) if periodic else (0.0, 0.0, -9.81))
     g = make_gpu(wl, precision=precision)
