@@ -192,6 +192,20 @@ int grains_gpu_set_neighbor_list(grains_gpu_system* sys, int capacity, double sk
  * so far.  No counterpart in the reference. */
 int grains_gpu_neighbor_stats(grains_gpu_system* sys, double* skin, long* builds);
 
+/* Contact statistics of the time loop (AppCollision::addToContactsFeatures,
+ * AppCollision.cpp:159-189, reported by Grains::Simulation,
+ * Grains.cpp:228-240): off by default; turning them on clears them.  A
+ * particle pair counts as often per force evaluation as the reference meets
+ * it: once, or once per pair of its periodic images (master and clones in
+ * the ghost layers) that exist together; a particle-obstacle pair once
+ * (the reference's clone-obstacle meetings are not counted).  out[0] maximum overlap,
+ * out[1] mean overlap, out[2] time of the maximum (first reached), out[3]
+ * mean GJK iterations (closed-form contact points count 1), out[4] overlaps
+ * counted, out[5] closest-point evaluations with iterations.  The means are
+ * sums over counts (the reference updates a running mean). */
+int grains_gpu_set_contact_stats(grains_gpu_system* sys, int on);
+int grains_gpu_contact_stats(grains_gpu_system* sys, double* out);
+
 /* Contact records of the last force evaluation: pairs (receiver id, partner
  * id), one per particle that received a contact force from the partner
  * (obstacle partners have negative ids).  Recording is off by default. */

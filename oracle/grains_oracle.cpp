@@ -223,7 +223,7 @@ struct System {
   // diagnostics
   std::vector<PairRec> pairs;
   bool record_pairs;
-  double overlap_max, overlap_mean, overlap_count;
+  double overlap_max, overlap_mean, overlap_count, time_overlap_max, gjk_iter_mean, gjk_count;
   long n_recovered;
   int err;
   int err_id0, err_id1;
@@ -356,7 +356,8 @@ int buildGrid(System* s, double cellsize) {
 struct Contact {
   V3 P, ov;
   double d;
-  bool any;  // false = PointNoContact
+  bool any;       // false = PointNoContact
+  int nit = -1;   // PointContact iteration count: GJK's, else 1 for a contact point
 };
 
 // ClosestPointSPHERESPHERE (RigidBodyWithCrust.cpp:596-640); returns false on
@@ -683,6 +684,7 @@ bool closestPointGJK(Geo const& A, Geo const& B, Contact& c) {
   double distance = gjkClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb);
   ++g_gjk_calls;
   g_gjk_iters += nb;
+  c.nit = nb;
   if (distance < EPSILON) return false;
   pa = applyTf(a2w, pa);
   pb = applyTf(b2w, pb);
@@ -704,6 +706,7 @@ bool closestPointRecoverGJK(Geo const& A, Geo const& B, Contact& c) {
   V3 pa, pb;
   int nb = 0;
   double distance = gjkClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb);
+  c.nit = nb;
   if (distance < EPSILON) return false;
   pa = applyTf(a2w, pa);
   pb = applyTf(b2w, pb);
@@ -811,12 +814,20 @@ void forcesPO(HookeParams const& m, Part& p0, Obs& o, Contact const& ci) {
   }
 }
 
-void contactStats(System* s, double d) {  // AppCollision::addToContactsFeatures (AppCollision.cpp:159-189)
-  if (d < 0.) {
-    double ov = std::fabs(d);
+void contactStats(System* s, Contact const& c) {  // AppCollision::addToContactsFeatures (AppCollision.cpp:159-189)
+  if (c.any && c.d < 0.) {
+    double ov = std::fabs(c.d);
     s->overlap_mean = (s->overlap_mean * s->overlap_count + ov) / (s->overlap_count + 1.);
     s->overlap_count += 1.;
-    if (ov > s->overlap_max) s->overlap_max = ov;
+    if (ov > s->overlap_max) {
+      s->overlap_max = ov;
+      s->time_overlap_max = s->time;
+    }
+  }
+  int nit = c.any ? (c.nit >= 0 ? c.nit : 1) : 0;
+  if (nit) {
+    s->gjk_iter_mean = (s->gjk_iter_mean * s->gjk_count + double(nit)) / (s->gjk_count + 1.);
+    s->gjk_count += 1.;
   }
 }
 
@@ -843,7 +854,7 @@ int interactPP(System* s, int ia, int ib) {
       s->n_recovered++;
     }
   }
-  contactStats(s, c.any ? c.d : 1.);
+  contactStats(s, c);
   if (c.any && c.d < 0.) {
     HookeParams* m = model_for(s, A.mat, B.mat);
     if (!m) return CONFIG_ERROR;
@@ -886,7 +897,7 @@ int interactPO(System* s, int io, int ip) {
       s->n_recovered++;
     }
   }
-  contactStats(s, c.any ? c.d : 1.);
+  contactStats(s, c);
   if (c.any && c.d < 0.) {
     HookeParams* m = model_for(s, A.mat, o.mat);
     if (!m) return CONFIG_ERROR;
@@ -1295,6 +1306,7 @@ void* orc_new(const double* origin, const double* dmax, const int* periodic, dou
   s->initialized = false;
   s->record_pairs = true;
   s->overlap_max = s->overlap_mean = s->overlap_count = 0.;
+  s->time_overlap_max = s->gjk_iter_mean = s->gjk_count = 0.;
   s->n_recovered = 0;
   s->err = 0;
   s->err_id0 = s->err_id1 = 0;
@@ -1764,6 +1776,10 @@ void orc_get_stats(void* h, double* out) {
   out[2] = (double)s->n_recovered;
   out[3] = (double)s->err_id0;
   out[4] = (double)s->err_id1;
+  out[5] = s->time_overlap_max;
+  out[6] = s->gjk_iter_mean;
+  out[7] = s->overlap_count;
+  out[8] = s->gjk_count;
 }
 
 // Overwrites master states (id order) and re-links, as a reload would; clones

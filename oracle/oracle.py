@@ -248,10 +248,11 @@ class Oracle:
         return np.frombuffer(buf, dtype=np.int32).reshape(-1, 3).copy()
 
     def stats(self):
-        out = np.zeros(5)
+        out = np.zeros(9)
         lib().orc_get_stats(self.h, _ptr(out))
         return {"overlap_max": out[0], "overlap_mean": out[1], "recovered": out[2], "err_id0": out[3],
-                "err_id1": out[4]}
+                "err_id1": out[4], "time_overlap_max": out[5], "gjk_iter_mean": out[6], "n_overlaps": out[7],
+                "n_evaluations": out[8]}
 
     def set_state(self, x=None, v=None, w=None, q=None, recompute=True):
         keep = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, v, w, q)]

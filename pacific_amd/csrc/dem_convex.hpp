@@ -251,13 +251,14 @@ DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B,
 // General ClosestPoint path.  Returns 1 with the contact features (the
 // caller tests d < 0), 0 when the circumscribed spheres do not meet, -1 on
 // ContactError (the crust-shrunk bodies intersect).
-DEM_HD int closest_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d) {
+DEM_HD int closest_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d, int* iters = nullptr) {
   d3 gcagcb = A.o - B.o;
   if (!(norm(gcagcb) < A.circR + B.circR)) return 0;
   Tf a2w = tf_scaled(A, A.scaling), b2w = tf_scaled(B, B.scaling);
   d3 pa, pb;
   int it;
   double distance = gjk_closest(A, a2w, B, b2w, pa, pb, it);
+  if (iters) *iters = it;
   if (distance < kEps) return -1;
   pa = tf_apply(a2w, pa);
   pb = tf_apply(b2w, pb);
@@ -272,7 +273,7 @@ DEM_HD int closest_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, do
 }
 
 // ClosestPoint_ErreurHandling(factor 10): false if still intersecting
-DEM_HD bool recover_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d) {
+DEM_HD bool recover_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d, int* iters = nullptr) {
   d3 sa, sb;
   double* as[3] = {&sa.x, &sa.y, &sa.z};
   double* bs[3] = {&sb.x, &sb.y, &sb.z};
@@ -287,6 +288,7 @@ DEM_HD bool recover_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, d
   d3 pa, pb;
   int it;
   double distance = gjk_closest(A, a2w, B, b2w, pa, pb, it);
+  if (iters) *iters = it;
   if (distance < kEps) return false;
   pa = tf_apply(a2w, pa);
   pb = tf_apply(b2w, pb);

@@ -264,10 +264,15 @@ DEM_HD int sphere_box_particle(const ConvexGeo& bx, const ConvexGeo& sp, bool sp
 // RigidBodyWithCrust::ClosestPoint dispatch for a pair with at least one
 // non-spherical body.  1: features set (contact if d < 0); 0: no contact;
 // -1: ContactError.
-DEM_HD int closest_convex(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d) {
-  if (A.type == kShapeSphere && B.type == kShapeBox) return sphere_box_particle(B, A, true, P, ov, d);
-  if (A.type == kShapeBox && B.type == kShapeSphere) return sphere_box_particle(A, B, false, P, ov, d);
-  return closest_gjk(A, B, P, ov, d);
+// iters: iterations the reference's PointContact reports (GJK's count; 1
+// for the closed forms' contact points, RigidBodyWithCrust.cpp:635, 689, 715)
+DEM_HD int closest_convex(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d, int* iters = nullptr) {
+  int r;
+  if (A.type == kShapeSphere && B.type == kShapeBox) r = sphere_box_particle(B, A, true, P, ov, d);
+  else if (A.type == kShapeBox && B.type == kShapeSphere) r = sphere_box_particle(A, B, false, P, ov, d);
+  else return closest_gjk(A, B, P, ov, d, iters);
+  if (iters) *iters = 1;
+  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -564,10 +569,58 @@ __device__ __forceinline__ void wrap_and_bin(Kin& k, const Grid& G, const Sorted
   S.slot[s] = atomicAdd(&S.count[key], 1);
 }
 
+// Contact statistics of one thread (AppCollision::addToContactsFeatures,
+// AppCollision.cpp:159-189): overlaps of contacts, iterations of the
+// closest-point evaluations that report some.
+struct StatAcc {
+  double sum = 0., max = 0., cnt = 0., git = 0., gn = 0.;
+  // one evaluation of overlap d with nit iterations, met mult times
+  __device__ __forceinline__ void add(double d, int nit, int mult) {
+    if (nit) {
+      git += (double)nit * mult;
+      gn += mult;
+    }
+    if (d < 0.) {
+      const double o = -d;
+      sum += o * mult;
+      cnt += mult;
+      if (o > max) max = o;
+    }
+  }
+};
+
+// How many times the reference evaluates a particle pair: once for every
+// shift u (one periodic length or none per periodic axis) for which both
+// A + u and B + u + s exist, B seen at its image B + s -- a particle has a
+// clone across a periodic face when it lies in the cell next to the opposite
+// face, and clones meet masters and each other in the ghost layers
+// (LinkedCell.cpp:1765-2398, :1019-1103).
+__device__ __forceinline__ int image_evaluations(const Grid& G, const d3& xa, const d3& xb, const int sh[3]) {
+  const double a3[3] = {xa.x, xa.y, xa.z}, b3[3] = {xb.x, xb.y, xb.z};
+  int m = 1;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!G.periodic[a]) continue;
+    const int ca = min(max(ref_cell(a3[a], G, a) - G.off[a], 0), G.n[a] - 1);
+    const int cb = min(max(ref_cell(b3[a], G, a) - G.off[a], 0), G.n[a] - 1);
+    // clone at +L from the first cell, at -L from the last one
+    auto exists = [&](int c, int u) { return u == 0 || (u > 0 ? c == 0 : c == G.n[a] - 1); };
+    int k = 0;
+    for (int u = -1; u <= 1; ++u) {
+      const int v = u + sh[a];
+      if (v < -1 || v > 1) continue;
+      k += exists(ca, u) && exists(cb, v);
+    }
+    m *= k;
+  }
+  return m;
+}
+
 struct Self {
   d3 x, v, w;
   double r, m, c, req;
   int mat, id;
+  StatAcc* acc = nullptr;  // contact statistics, when collected
 };
 
 __device__ __forceinline__ void record_pair(const StepParams& sp, DevStatus* st, int* pairs, int a, int b,
@@ -727,20 +780,23 @@ __device__ __forceinline__ void for_each_candidate(const Sorted& S, const Grid& 
 // contact (overlap < 0), the visitor.  Particle::InterAction
 // (Particle.cpp:477-545) with the ContactError recovery of
 // RigidBodyWithCrust.cpp:534-590.
+// In the statistics a pair counts on the side of its lower id, as often as
+// the reference meets it (image_evaluations; sh = the image code of xo).
 template <class V, bool SHAPES>
-__device__ __forceinline__ void narrow_visit(const Particles& P, DevStatus* st, const Self& me, V& vis,
-                                             const ConvexGeo* mg, int cx, int k2, int s2i, int pid2, d3 xo,
-                                             double ro, d3 AB, double nab) {
+__device__ __forceinline__ void narrow_visit(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
+                                             V& vis, const ConvexGeo* mg, int cx, int k2, int s2i, int pid2, d3 xo,
+                                             double ro, d3 AB, double nab, const d3& xb, const int sh[3]) {
   const double co = P.crust[s2i];
   d3 Pc, ov;
   double d;
+  int nit = 1;
   if (SHAPES && (mg->type != kShapeSphere ||
                  (P.shape[s2i] >= 0 && P.shapes[P.shape[s2i]].type != kShapeSphere))) {
     ConvexGeo og = convex_of(P, s2i, xo, cx);
-    int res = closest_convex(*mg, og, Pc, ov, d);
+    int res = closest_convex(*mg, og, Pc, ov, d, &nit);
     if (res == 0) return;
     if (res < 0) {
-      if (!recover_gjk(*mg, og, Pc, ov, d)) {
+      if (!recover_gjk(*mg, og, Pc, ov, d, &nit)) {
         raise_error(st, kContactError, me.id, pid2);
         return;
       }
@@ -757,6 +813,7 @@ __device__ __forceinline__ void narrow_visit(const Particles& P, DevStatus* st, 
       atomicAdd(&st->n_recovered, 1);
     }
   }
+  if (me.acc && me.id < pid2) me.acc->add(d, nit, image_evaluations(G, me.x, xb, sh));
   if (!(d < 0.)) return;
   vis.pair(k2, s2i, pid2, xo, ro, Pc, ov, d);
 }
@@ -780,7 +837,9 @@ __device__ __forceinline__ void visit_candidate(const Particles& P, const Sorted
   double nab = sqrt(s2);
   const int s2i = S.order[k2];
   if (s2i < slot_lo || s2i >= slot_hi) return;
-  narrow_visit<V, SHAPES>(P, st, me, vis, mg, cx, k2, s2i, S.sid[k2], xo, ro, AB, nab);
+  const int sh[3] = {ix, iy, iz};
+  narrow_visit<V, SHAPES>(P, G, st, me, vis, mg, cx, k2, s2i, S.sid[k2], xo, ro, AB, nab, d3{q2[0], q2[1], q2[2]},
+                          sh);
 }
 
 // Particle s2i at its current position (position buffer cx), seen at its
@@ -802,13 +861,15 @@ __device__ __forceinline__ d3 nearest_image(const Grid& G, d3 xo, const d3& x) {
 
 template <class V, bool SHAPES>
 __device__ __forceinline__ void visit_current_at(const Particles& P, const Grid& G, DevStatus* st, const Self& me,
-                                                 V& vis, const ConvexGeo* mg, int cx, int s2i, d3 xo, double ro) {
-  xo = nearest_image(G, xo, me.x);
+                                                 V& vis, const ConvexGeo* mg, int cx, int s2i, d3 x2, double ro) {
+  const d3 xo = nearest_image(G, x2, me.x);
   d3 AB = xo - me.x;
   double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
   double R = me.r + ro;
   if (s2 > R * R * 1.000000000001) return;
-  narrow_visit<V, SHAPES>(P, st, me, vis, mg, cx, -1, s2i, P.id[s2i], xo, ro, AB, sqrt(s2));
+  const int sh[3] = {xo.x < x2.x ? -1 : (xo.x > x2.x ? 1 : 0), xo.y < x2.y ? -1 : (xo.y > x2.y ? 1 : 0),
+                     xo.z < x2.z ? -1 : (xo.z > x2.z ? 1 : 0)};
+  narrow_visit<V, SHAPES>(P, G, st, me, vis, mg, cx, -1, s2i, P.id[s2i], xo, ro, AB, sqrt(s2), x2, sh);
 }
 
 template <class V, bool SHAPES>
@@ -831,12 +892,13 @@ __device__ __forceinline__ void visit_obstacles(const BoxObstacle* __restrict__ 
     if (!probable) continue;
     d3 Pc, ov;
     double d;
+    int nit = 1;
     if (SHAPES && mg->type != kShapeSphere) {
       ConvexGeo og = convex_of_obstacle(B);
-      int res = closest_convex(*mg, og, Pc, ov, d);
+      int res = closest_convex(*mg, og, Pc, ov, d, &nit);
       if (res == 0) continue;
       if (res < 0) {
-        if (!recover_gjk(*mg, og, Pc, ov, d)) {
+        if (!recover_gjk(*mg, og, Pc, ov, d, &nit)) {
           raise_error(st, kContactError, me.id, B.id);
           continue;
         }
@@ -853,6 +915,7 @@ __device__ __forceinline__ void visit_obstacles(const BoxObstacle* __restrict__ 
         atomicAdd(&st->n_recovered, 1);
       }
     }
+    if (me.acc) me.acc->add(d, nit, 1);
     if (!(d < 0.)) continue;
     vis.wall(o, Pc, ov, d);
   }

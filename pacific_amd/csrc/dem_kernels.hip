@@ -284,15 +284,10 @@ __device__ __forceinline__ void sphere_tail(const Particles& P, const Sorted& S,
 }
 
 template <int MODE, bool MIXED, bool SHAPES>
-__global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
-                                                const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
-                                                int* pairs, NbrList L, VerletState VS, NbrList CL) {
-  // Spheres: one thread per storage index, so that the particle's own state
-  // is read and written in contiguous runs; its sorted slot locates the cell
-  // data.  Non-spherical bodies: one thread per sorted slot, so that a wave
-  // holds neighbouring bodies (GJK dominates, and neighbours share branches).
-  const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= sp.n) return;
+__device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, const Grid& G,
+                                          const BoxObstacle* __restrict__ obs, const HookeModel* __restrict__ models,
+                                          const StepParams& sp, DevStatus* st, int* pairs, const NbrList& L,
+                                          const VerletState& VS, const NbrList& CL, int t, StatAcc* acc) {
   const int cx = sp.cur & 1, cv = (sp.cur >> 1) & 1;
   const int s = SHAPES ? S.order[t] : t;
   const int k = SHAPES ? t : S.inv[t];
@@ -315,6 +310,7 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   me.c = P.crust[s];
   me.req = P.req[s];
   me.mat = P.mat[s];
+  me.acc = acc;
   const double wf = P.wfac[s];
   // InitializeForce: torsor = weight at the centre (Particle.cpp:1408-1415)
   d3 F{sp.g[0] * wf, sp.g[1] * wf, sp.g[2] * wf};
@@ -361,6 +357,90 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
     return;
   }
   sphere_tail<MODE>(P, S, G, sp, st, VS, s, cx, cv, kin, me.m, me.c, me.id, cf, I, F, T);
+}
+
+// Per-block sums of the contact statistics (sum and max of the overlaps,
+// their count, GJK iterations and evaluations), in a fixed order.
+__device__ __forceinline__ void block_stats(const StatAcc& a, double* out) {
+  __shared__ double part[4][5];
+  double v[5] = {a.sum, a.max, a.cnt, a.git, a.gn};
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const double o = __shfl_down(v[j], off);
+      v[j] = j == 1 ? fmax(v[j], o) : v[j] + o;
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int j = 0; j < 5; ++j) part[w][j] = v[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int j = 0; j < 5; ++j) {
+      double r = part[0][j];
+      for (int k = 1; k < nw; ++k) r = j == 1 ? fmax(r, part[k][j]) : r + part[k][j];
+      out[j] = r;
+    }
+  }
+}
+
+// Spheres: one thread per storage index, so that the particle's own state is
+// read and written in contiguous runs; its sorted slot locates the cell data.
+// Non-spherical bodies: one thread per sorted slot, so that a wave holds
+// neighbouring bodies (GJK dominates, and neighbours share branches).
+// STATS: contact statistics may be collected (sp.stats set).  Spheres have a
+// separate instantiation without them, so that the common launch carries
+// none of their cost; the GJK kernels, heavy to compile, check at run time.
+template <int MODE, bool MIXED, bool SHAPES, bool STATS>
+__global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
+                                                const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
+                                                int* pairs, NbrList L, VerletState VS, NbrList CL) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (!STATS) {
+    if (t < sp.n) force_one<MODE, MIXED, SHAPES>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, nullptr);
+    return;
+  }
+  StatAcc acc;
+  if (t < sp.n)
+    force_one<MODE, MIXED, SHAPES>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, sp.stats ? &acc : nullptr);
+  if (sp.stats) block_stats(acc, sp.stats + 5 * (size_t)blockIdx.x);
+}
+
+// Running totals of the contact statistics after a force launch of nb
+// blocks: tot = {overlap sum, overlap max, time of the max, contacts, GJK
+// iterations, evaluations with iterations}; the maximum keeps the first
+// time it was reached (AppCollision.cpp:174-178).
+__global__ __launch_bounds__(256) void k_stats_fold(const double* __restrict__ part, int nb, double* tot,
+                                                    double time) {
+  StatAcc a;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const double* p = part + 5 * (size_t)i;
+    a.sum += p[0];
+    a.max = fmax(a.max, p[1]);
+    a.cnt += p[2];
+    a.git += p[3];
+    a.gn += p[4];
+  }
+  __shared__ double r[5];
+  block_stats(a, r);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    tot[0] += r[0];
+    if (r[1] > tot[1]) {
+      tot[1] = r[1];
+      tot[2] = time;
+    }
+    tot[3] += r[2];
+    tot[4] += r[3];
+    tot[5] += r[4];
+  }
+}
+
+hipError_t launch_stats_fold(const double* part, int n, double* tot, double time, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stats_fold, dim3(1), dim3(256), 0, stream, part, (n + 255) / 256, tot, time);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -517,12 +597,18 @@ template <int MODE, bool MIXED>
 static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
                             const HookeModel* models, const StepParams& sp, DevStatus* st, int* pairs,
                             const NbrList& L, const VerletState& V, const NbrList& CL, hipStream_t stream) {
-  if (P.shape)
-    hipLaunchKernelGGL((k_forces<MODE, MIXED, true>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs, L, V, CL);
-  else
-    hipLaunchKernelGGL((k_forces<MODE, MIXED, false>), dim3(nblk(sp.n, 256)), dim3(256), 0, stream, P, S, G, obs,
-                       models, sp, st, pairs, L, V, CL);
+  const dim3 grid(nblk(sp.n, 256)), block(256);
+  if (P.shape) {
+    hipLaunchKernelGGL((k_forces<MODE, MIXED, true, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
+                       pairs, L, V, CL);
+  } else {
+    if (sp.stats)
+      hipLaunchKernelGGL((k_forces<MODE, MIXED, false, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
+                         pairs, L, V, CL);
+    else
+      hipLaunchKernelGGL((k_forces<MODE, MIXED, false, false>), grid, block, 0, stream, P, S, G, obs, models, sp,
+                         st, pairs, L, V, CL);
+  }
 }
 
 hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, const VerletState& V, int n, int cx,

@@ -29,6 +29,9 @@ hipError_t launch_clear(int* p, int n, hipStream_t stream);
 // contacts of this step from the Verlet list L into CL
 hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, const VerletState& V, int n, int cx,
                           const NbrList& L, const NbrList& CL, hipStream_t stream);
+// folds the per-block contact statistics of the last force launch over n
+// particles into the running totals (6 doubles), at the given time
+hipError_t launch_stats_fold(const double* part, int n, double* tot, double time, hipStream_t stream);
 // neighbour candidates of every sorted particle into L
 hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream);
 // force sweep over the candidates of L (L.count == nullptr: 27-cell sweep)

@@ -126,6 +126,9 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, track(this, &d_status, 1));
   if (verlet)
     for (int a = 0; a < 3; ++a) HIPCHECK(this, track(this, &V.xb[a], n));
+  HIPCHECK(this, track(this, &d_stat_part, 5 * ((n + 255) / 256) + 5));
+  HIPCHECK(this, track(this, &d_stat_tot, 8));
+  HIPCHECK(this, hipMemsetAsync(d_stat_tot, 0, 8 * sizeof(double), stream));
   if (verlet) {
     HIPCHECK(this, track(this, &d_ct_count, n));
     HIPCHECK(this, track(this, &d_ct, (size_t)kContactCap * n));
@@ -291,6 +294,8 @@ StepParams grains_gpu_system::params() const {
   sp.pair_cap = pair_cap;
   sp.scheme = cfg.scheme;
   sp.first_step = 0;
+  // the adjoint's replays are not counted again
+  sp.stats = stats_on && !recording ? d_stat_part : nullptr;
   return sp;
 }
 
@@ -355,6 +360,7 @@ int grains_gpu_system::forces(int mode, int first_step) {
   HIPCHECK(this, launch_forces(mode, cfg.precision == GRAINS_MIXED, P, S, grid, d_obs, d_models, sp, d_status,
                                d_pairs, L, vstate(next_stamp()), CL, stream));
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
+  if (sp.stats) HIPCHECK(this, launch_stats_fold(d_stat_part, nparticles, d_stat_tot, force_time, stream));
   if (mode == kModeForcesKick) cur ^= 2;
   if (mode == kModeFused) cur ^= 3;
   return GRAINS_OK;
@@ -385,19 +391,26 @@ int grains_gpu_system::enqueue_steps_impl(int nsteps) {
     // Move(dt/2, dt) -> LinkUpdate -> forces -> advanceVelocity(dt/2); the
     // forces, the second half kick and the next Move share one kernel.
     StepParams sp = params();
+    // forces of step j are evaluated at the time the step reaches (the
+    // reference adds dt to m_time before them, Grains.cpp:290)
+    force_time = time;
     HIPCHECK(this, launch_move(P, S, grid, sp, d_status, 0.5 * cfg.dt, cfg.dt, vstate(next_stamp()), stream));
     if ((e = bin_launch())) return e;
     if (recording && (e = record_state())) return e;
     for (int i = 1; i < nsteps; ++i) {
+      force_time += cfg.dt;
       if ((e = forces(kModeFused, 0))) return e;
       if ((e = bin_launch())) return e;
       if (recording && (e = record_state())) return e;
     }
+    force_time += cfg.dt;
     if ((e = forces(kModeForcesKick, 0))) return e;
   } else {
     // forces -> Move(dt, dt) -> LinkUpdate (Grains.cpp:365-377)
+    force_time = time;
     for (int i = 0; i < nsteps; ++i) {
       int first = (cfg.scheme == GRAINS_ADAMS_BASHFORTH2 && steps_done + i == 0) ? 1 : 0;
+      force_time += cfg.dt;
       if ((e = forces(kModeFused, first))) return e;
       if ((e = bin_launch())) return e;
     }
@@ -455,6 +468,7 @@ int grains_gpu_system::initialize() {
   initialized = true;
   // LeapFrog: accelerations at the initial time (Grains.cpp:169-174)
   if (cfg.scheme == GRAINS_LEAPFROG) {
+    force_time = time;
     if ((e = forces(kModeForcesOnly, 0))) return e;
   }
   return check_status();
@@ -822,7 +836,11 @@ int grains_gpu_synchronize(grains_gpu_system* s) {
 int grains_gpu_compute_forces(grains_gpu_system* s) {
   if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
   HIPCHECK(s, hipSetDevice(s->device));
+  // an evaluation outside the time loop does not enter the statistics
+  const bool on = s->stats_on;
+  s->stats_on = false;
   int e = s->forces(kModeForcesOnly, 0);
+  s->stats_on = on;
   if (e) return e;
   return s->check_status();
 }
@@ -970,6 +988,32 @@ int grains_gpu_neighbor_stats(grains_gpu_system* s, double* skin, long* builds) 
   if (e) return e;
   if (skin) *skin = s->verlet ? s->V.skin : 0.;
   if (builds) *builds = s->verlet ? s->last_status.n_rebuild : 0;
+  return GRAINS_OK;
+}
+
+int grains_gpu_set_contact_stats(grains_gpu_system* s, int on) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  s->stats_on = on != 0;
+  if (s->initialized && on) {
+    HIPCHECK(s, hipSetDevice(s->device));
+    HIPCHECK(s, hipMemsetAsync(s->d_stat_tot, 0, 8 * sizeof(double), s->stream));
+  }
+  return GRAINS_OK;
+}
+
+int grains_gpu_contact_stats(grains_gpu_system* s, double* out) {
+  if (!s || !s->initialized || !out) return GRAINS_ARGUMENT_ERROR;
+  HIPCHECK(s, hipSetDevice(s->device));
+  double t[8];
+  HIPCHECK(s, hipMemcpyAsync(t, s->d_stat_tot, sizeof(t), hipMemcpyDeviceToHost, s->stream));
+  int e = s->check_status();
+  if (e) return e;
+  out[0] = t[1];
+  out[1] = t[3] > 0. ? t[0] / t[3] : 0.;
+  out[2] = t[2];
+  out[3] = t[5] > 0. ? t[4] / t[5] : 0.;
+  out[4] = t[3];
+  out[5] = t[5];
   return GRAINS_OK;
 }
 

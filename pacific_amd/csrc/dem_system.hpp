@@ -87,6 +87,12 @@ struct grains_gpu_system {
     return verlet ? dem::NbrList{d_ct_count, d_ct, kContactCap, nparticles} : dem::NbrList{};
   }
   int stamp = 0;
+  // contact statistics (AppCollision): per-block partials of the last force
+  // launch and running totals; force_time = time of the next force launch
+  bool stats_on = false;
+  double* d_stat_part = nullptr;
+  double* d_stat_tot = nullptr;
+  double force_time = 0.;
   int next_stamp() { return stamp = stamp >= (1 << 30) ? 1 : stamp + 1; }
   dem::VerletState vstate(int st) const {
     dem::VerletState v = V;

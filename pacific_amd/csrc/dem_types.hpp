@@ -161,6 +161,7 @@ struct StepParams {
   int pair_cap;
   int scheme;
   int first_step;     // Adams-Bashforth first step (Euler start)
+  double* stats;      // contact statistics: 5 partial sums per force block (nullptr: not collected)
 };
 
 }  // namespace dem

@@ -46,7 +46,7 @@ EXPORTS = [
     "grains_gpu_get_basis", "grains_gpu_set_basis", "grains_gpu_create_coupled", "grains_gpu_fluid_step",
     "grains_gpu_to_fluid", "grains_gpu_set_velocities", "grains_gpu_count_rigid_bodies",
     "grains_gpu_time_interval", "grains_gpu_closest_point", "grains_gpu_set_neighbor_list",
-    "grains_gpu_neighbor_stats",
+    "grains_gpu_neighbor_stats", "grains_gpu_set_contact_stats", "grains_gpu_contact_stats",
 ]
 
 
@@ -79,6 +79,8 @@ def lib():
         L.grains_gpu_set_record_pairs.argtypes = [vp, ctypes.c_int]
         L.grains_gpu_set_neighbor_list.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.grains_gpu_neighbor_stats.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
+        L.grains_gpu_set_contact_stats.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_contact_stats.argtypes = [vp, dp]
         L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
         L.grains_gpu_num_particles.argtypes = [vp]
         L.grains_gpu_num_particles.restype = sz
@@ -396,6 +398,18 @@ class GrainsGPU:
         skin, builds = ctypes.c_double(), ctypes.c_long()
         self._check(lib().grains_gpu_neighbor_stats(self.h, ctypes.byref(skin), ctypes.byref(builds)))
         return skin.value, builds.value
+
+    def contact_stats(self, on=None):
+        """With on=True/False: turns the contact statistics on (cleared) or
+        off.  Without: dict(overlap_max, overlap_mean, time_overlap_max,
+        gjk_iter_mean, n_overlaps, n_evaluations)."""
+        if on is not None:
+            self._check(lib().grains_gpu_set_contact_stats(self.h, 1 if on else 0))
+            return None
+        out = np.zeros(6)
+        self._check(lib().grains_gpu_contact_stats(self.h, _d(out)))
+        return {"overlap_max": out[0], "overlap_mean": out[1], "time_overlap_max": out[2],
+                "gjk_iter_mean": out[3], "n_overlaps": out[4], "n_evaluations": out[5]}
 
     def record_pairs(self, on=True):
         self._check(lib().grains_gpu_set_record_pairs(self.h, 1 if on else 0))

@@ -14,9 +14,10 @@ def make_oracle(wl, scheme=0, fluid_density=0.0):
     return o
 
 
-def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None, skin=None):
+def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None, skin=None, stats=False):
     """nbr: candidates per particle (0: cell search in the force sweep);
-    skin: Verlet skin (0: per-step list); None: the library defaults."""
+    skin: Verlet skin (0: per-step list); None: the library defaults;
+    stats: collect contact statistics from the first force evaluation."""
     from pacific_amd import GrainsGPU
     g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], fluid_density,
                   wl["dt"], scheme, precision)
@@ -25,6 +26,8 @@ def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None
         g.neighbor_list(32 if nbr is None else nbr, -1.0 if skin is None else skin)
     if record:
         g.record_pairs(True)
+    if stats:
+        g.contact_stats(True)
     g.init()
     return g
 
