@@ -142,6 +142,18 @@ enum ErrCode { OK = 0, CONTACT_ERROR = 1, MOTION_ERROR = 2, SIMULATION_ERROR = 3
 struct HookeParams {  // HookeContactForceModel.cpp:11-20
   double kn, en, etat, muc, kr, beta;
   bool defined;
+  // HookeMemoryContactForceModel.cpp:11-29: tangential spring kt, rolling
+  // friction mur (on when != 0) with its damping factor etarpf, threshold epst
+  bool memory = false;
+  double kt = 0., mur = 0., etarpf = 0., epst = 0.;
+};
+
+// One entry of Component::m_contactMap (Component.cpp:634-690): touched
+// during the current force evaluation, cumulative kt * tangential motion,
+// previous normal, cumulative rolling spring torque.
+struct ContactMem {
+  bool flag;
+  V3 kd, pn, sr;
 };
 
 struct Part {
@@ -152,6 +164,7 @@ struct Part {
   V3 x, v, w, wb;  // centre, translational, space-fixed and body-fixed angular velocity
   Q q;
   V3 F, T, refp;   // torsor (Torsor.cpp:160-190)
+  std::map<int, ContactMem> mem;  // contact memory by partner id (elementary ids are 1)
   double radius, crust, mass, inertia[6], req, density;
   int mat;
   V3 weight;
@@ -773,6 +786,95 @@ void hookeForces(HookeParams const& m, Body const& p0, Body const& p1, Contact c
     delM = V3(0., 0., 0.);
 }
 
+// Quaternion::setRotFromTwoVectors (Quaternion.cpp:263-286)
+Q rotFromTwoVectors(V3 const& u, V3 const& v) {
+  double norm_u_norm_v = std::sqrt(dot(u, u) * dot(v, v));
+  double real_part = norm_u_norm_v + dot(u, v);
+  V3 vect;
+  if (real_part < 1.e-6 * norm_u_norm_v) {
+    real_part = 0.;
+    vect = std::fabs(u[0]) > std::fabs(u[2]) ? V3(-u[1], u[0], 0.) : V3(0., -u[2], u[1]);
+  } else {
+    vect = cross(u, v);
+  }
+  Q qq(vect, real_part);
+  double nq = std::sqrt(qq.v[0] * qq.v[0] + qq.v[1] * qq.v[1] + qq.v[2] * qq.v[2] + qq.w * qq.w);
+  double d = 1. / nq;
+  return Q(d * qq.v, d * qq.w);
+}
+// Quaternion::rotateVector (Quaternion.cpp:293-298)
+V3 rotateVector(Q const& q, V3 const& v) {
+  double nv = Norm(q.v);
+  return ((q.w * q.w - nv * nv) * v + (2. * dot(v, q.v)) * q.v) + (2. * q.w) * cross(q.v, v);
+}
+
+// HookeMemoryContactForceModel::computeTangentialVector (:54-65)
+V3 memTangent(HookeParams const& m, double n_t, V3 const& ut, V3 const& kdelta) {
+  if (Norm(ut) > 0.001) return -ut / Norm(ut);
+  V3 a = kdelta + n_t * ut;
+  if ((Norm(kdelta) > m.epst || Norm(n_t * ut) > m.epst) && Norm(a) > m.epst) return -a / Norm(a);
+  return V3(0., 0., 0.);
+}
+
+// HookeMemoryContactForceModel::performForcesCalculus (:71-193): mem0 is
+// p0's entry for p1 (created zero when the contact is new); returns the
+// entry p1 receives for p0.
+ContactMem hookeMemoryForces(HookeParams const& m, Body const& p0, Body const& p1, Contact const& ci, double dt,
+                             bool existed, ContactMem& mem0, V3& delFN, V3& delFT, V3& delM) {
+  V3 P = ci.P;
+  V3 pen = ci.ov;
+  V3 normal(pen);
+  normal /= Norm(normal);
+  roundv(normal);
+  V3 tmpV = velAt(p0, P) - velAt(p1, P);
+  V3 v_n = normal * dot(tmpV, normal);
+  V3 v_t = tmpV - v_n;
+  delFN = m.kn * pen;
+  double avmass = 1. / (1. / p0.mass + 1. / p1.mass);
+  double gamman = -2. * m.beta * std::sqrt(avmass * m.kn);
+  delFN -= gamman * v_n;
+  double normFN = Norm(delFN);
+  double etat = m.etat == -1. ? -m.beta * std::sqrt(m.kn / avmass) : m.etat;
+  double gammat = 2. * etat * avmass;
+  V3 viscousFT = -gammat * v_t;
+  Q qrot(V3(0., 0., 0.), 0.);
+  if (existed) {
+    qrot = rotFromTwoVectors(mem0.pn, normal);
+    mem0.kd = rotateVector(qrot, mem0.kd);
+  }
+  mem0.kd += (dt * m.kt) * v_t;
+  mem0.pn = normal;
+  V3 tij = memTangent(m, gammat, v_t, mem0.kd);
+  double normFT = Norm(-mem0.kd + viscousFT);
+  if (normFT <= m.muc * normFN) {
+    delFT = Norm(-mem0.kd + viscousFT) * tij;
+  } else {
+    if (m.kt) mem0.kd = ((-m.muc) * normFN * tij + viscousFT) / m.kt;
+    delFT = (m.muc * normFN) * tij;
+  }
+  if (m.mur) {
+    double Req = 1. / (1. / p0.req + 1. / p1.req);
+    double kr = 3. * m.kn * m.mur * m.mur * Req * Req;
+    double etar = 3. * gamman * m.mur * m.mur * Req * Req;
+    double max_normMk = m.mur * Req * normFN;
+    V3 wrel = p0.w - p1.w;
+    if (existed) mem0.sr = rotateVector(qrot, mem0.sr);
+    mem0.sr -= (kr * dt) * wrel;
+    double normMk = Norm(mem0.sr);
+    if (normMk > max_normMk) mem0.sr *= max_normMk / normMk;
+    delM = mem0.sr - (m.etarpf * etar) * wrel;
+  } else {
+    delM = V3(0., 0., 0.);
+  }
+  mem0.flag = true;
+  ContactMem m1;
+  m1.flag = true;
+  m1.kd = -mem0.kd;
+  m1.pn = -normal;
+  m1.sr = -mem0.sr;
+  return m1;
+}
+
 // Torsor::addForce(point, f) (Torsor.cpp:184-190)
 inline void torsorAdd(V3& F, V3& T, V3 const& refp, V3 const& P, V3 const& f) {
   F += f;
@@ -788,29 +890,63 @@ Body bodyOf(Part const& p) {
 
 // HookeContactForceModel::computeForces (HookeContactForceModel.cpp:121-163)
 // for a particle pair.
+// Component::getContactMemory with createContact (Component.cpp:634-674)
+bool getContactMemory(Part& p, int id, ContactMem*& e) {
+  auto it = p.mem.find(id);
+  if (it != p.mem.end()) {
+    it->second.flag = true;
+    e = &it->second;
+    return true;
+  }
+  ContactMem z;
+  z.flag = true;
+  z.kd = z.pn = z.sr = V3(0., 0., 0.);
+  e = &p.mem.emplace(id, z).first->second;
+  return false;
+}
+
 void forcesPP(System* s, HookeParams const& m, Part& p0, Part& p1, Contact const& ci) {
   V3 fn, ft, dm;
-  hookeForces(m, bodyOf(p0), bodyOf(p1), ci, fn, ft, dm);
   double coef = 1. / double(1);
+  if (m.memory) {
+    // HookeMemoryContactForceModel::computeForces (:200-247)
+    ContactMem* e0;
+    bool existed = getContactMemory(p0, p1.id, e0);
+    ContactMem e1 = hookeMemoryForces(m, bodyOf(p0), bodyOf(p1), ci, s->dt, existed, *e0, fn, ft, dm);
+    p1.mem[p0.id] = e1;  // Component::addDeplContactInMap (:680-690)
+    torsorAdd(p0.F, p0.T, p0.refp, ci.P, coef * (fn + ft));
+    if (m.mur) p0.T += dm * coef;
+    torsorAdd(p1.F, p1.T, p1.refp, ci.P, coef * (-fn - ft));
+    if (m.mur) p1.T += -dm * coef;
+    return;
+  }
+  hookeForces(m, bodyOf(p0), bodyOf(p1), ci, fn, ft, dm);
   torsorAdd(p0.F, p0.T, p0.refp, ci.P, coef * (fn + ft));
   if (m.kr) p0.T += dm * coef;
   torsorAdd(p1.F, p1.T, p1.refp, ci.P, coef * (-fn - ft));
   if (m.kr) p1.T += -dm * coef;
-  (void)s;
 }
 // particle (p0) - obstacle (p1); Obstacle::addForce skips clone partners
 // (Obstacle.cpp:575-596)
-void forcesPO(HookeParams const& m, Part& p0, Obs& o, Contact const& ci) {
+void forcesPO(System* s, HookeParams const& m, Part& p0, Obs& o, Contact const& ci) {
   Body b1;
   b1.x = o.centre; b1.v = V3(); b1.w = V3(); b1.mass = OBSTACLE_MASS; b1.req = o.req;
   V3 fn, ft, dm;
-  hookeForces(m, bodyOf(p0), b1, ci, fn, ft, dm);
+  if (m.memory) {
+    // the obstacle's own map is never read back for the particle's force
+    ContactMem* e0;
+    bool existed = getContactMemory(p0, o.id, e0);
+    hookeMemoryForces(m, bodyOf(p0), b1, ci, s->dt, existed, *e0, fn, ft, dm);
+  } else {
+    hookeForces(m, bodyOf(p0), b1, ci, fn, ft, dm);
+  }
+  const bool rolling = m.memory ? m.mur != 0. : m.kr != 0.;
   double coef = 1. / double(1);
   torsorAdd(p0.F, p0.T, p0.refp, ci.P, coef * (fn + ft));
-  if (m.kr) p0.T += dm * coef;
+  if (rolling) p0.T += dm * coef;
   if (!p0.clone) {
     torsorAdd(o.F, o.T, o.centre, ci.P, coef * (-fn - ft));
-    if (m.kr) o.T += -dm * coef;
+    if (rolling) o.T += -dm * coef;
   }
 }
 
@@ -901,7 +1037,7 @@ int interactPO(System* s, int io, int ip) {
   if (c.any && c.d < 0.) {
     HookeParams* m = model_for(s, A.mat, o.mat);
     if (!m) return CONFIG_ERROR;
-    forcesPO(*m, A, o, c);
+    forcesPO(s, *m, A, o, c);
     if (s->record_pairs && !A.clone) s->pairs.push_back({A.id, o.id});
   }
   return OK;
@@ -941,6 +1077,9 @@ int computeForcesLC(System* s) {
 // AllComponents::InitializeForces (AllComponents.cpp:308-320) and
 // Particle::InitializeForce (Particle.cpp:1408-1415)
 int computeParticlesForceAndTorque(System* s) {
+  // AllComponents::setAllContactMapToFalse (AllComponents.cpp:2114-2124)
+  for (int ip : s->active)
+    for (auto& kv : s->pool[ip].mem) kv.second.flag = false;
   for (int ip : s->active) {
     Part& p = s->pool[ip];
     p.refp = p.x;
@@ -949,7 +1088,14 @@ int computeParticlesForceAndTorque(System* s) {
   }
   for (auto& o : s->obstacles) { o.F = V3(0., 0., 0.); o.T = V3(0., 0., 0.); }
   s->pairs.clear();
-  return computeForcesLC(s);
+  int e = computeForcesLC(s);
+  // AllComponents::updateAllContactMaps (:2132-2142): entries not met by
+  // this evaluation are dropped
+  for (int ip : s->active) {
+    auto& mm = s->pool[ip].mem;
+    for (auto it = mm.begin(); it != mm.end();) it = it->second.flag ? std::next(it) : mm.erase(it);
+  }
+  return e;
 }
 
 // ----------------------------------------------------------------------------
@@ -1318,6 +1464,22 @@ void* orc_new(const double* origin, const double* dmax, const int* periodic, dou
 void orc_free(void* h) { delete (System*)h; }
 
 // material indices are small integers; matB < 0 selects the Default model
+// HookeMemoryContactForceModel::defineParameters (:253-307): p = {kn, en,
+// etat, muc, kt, mur, etarpf, epst}
+int orc_set_memory_contact_model(void* h, int matA, int matB, const double* p) {
+  System* s = (System*)h;
+  HookeParams m;
+  m.kn = p[0]; m.en = p[1]; m.etat = p[2]; m.muc = p[3]; m.kr = 0.;
+  m.memory = true;
+  m.kt = p[4]; m.mur = p[5]; m.etarpf = p[6]; m.epst = p[7];
+  m.beta = hooke_beta(m.en);
+  m.defined = true;
+  if (matA < 0 || matB < 0) { s->default_model = m; return OK; }
+  s->models[matA * 64 + matB] = m;
+  s->models[matB * 64 + matA] = m;
+  return OK;
+}
+
 int orc_set_contact_model(void* h, int matA, int matB, double kn, double en, double etat, double muc, double kr) {
   System* s = (System*)h;
   HookeParams m;

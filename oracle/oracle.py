@@ -31,6 +31,7 @@ def lib():
         L.orc_new.argtypes = [dp, dp, ip, ctypes.c_double, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_free.argtypes = [vp]
         L.orc_set_contact_model.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 5
+        L.orc_set_memory_contact_model.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.orc_add_sphere.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                      dp, dp, dp, dp]
         L.orc_add_spheres.argtypes = [vp, ctypes.c_int, dp, dp, dp, ip, dp, dp, dp]
@@ -105,6 +106,11 @@ class Oracle:
 
     def contact_model(self, matA, matB, kn, en, muc, kr=0.0, etat=-1.0):
         lib().orc_set_contact_model(self.h, matA, matB, kn, en, etat, muc, kr)
+
+    def memory_contact_model(self, matA, matB, kn, en, muc, kt, etat=-1.0, mur=0.0, etarpf=0.0, epst=0.0):
+        """HookeMemory (tangential spring kt, rolling friction mur)."""
+        p = np.array([kn, en, etat, muc, kt, mur, etarpf, epst], np.float64)
+        lib().orc_set_memory_contact_model(self.h, matA, matB, p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
 
     def add_spheres(self, radius, density, crust, material, x, v=None, w=None):
         x = np.asarray(x, np.float64).reshape(-1, 3)

@@ -299,8 +299,13 @@ def silo(n=200_000, seed=4, width=0.1, orifice=12e-3, pitch=3.6e-3, crust=5e-5, 
 
 def build(system, wl):
     """Fills a GrainsGPU (or any object with the same methods) from a workload."""
-    for (ma, mb, kn, en, muc, kr, etat) in wl["models"]:
-        system.contact_model(ma, mb, kn, en, muc, kr=kr, etat=etat)
+    for m in wl["models"]:
+        if m[0] == "memory":  # ("memory", matA, matB, kn, en, muc, kt, etat, mur, etarpf, epst)
+            _, ma, mb, kn, en, muc, kt, etat, mur, etarpf, epst = m
+            system.memory_contact_model(ma, mb, kn, en, muc, kt, etat=etat, mur=mur, etarpf=etarpf, epst=epst)
+        else:
+            ma, mb, kn, en, muc, kr, etat = m
+            system.contact_model(ma, mb, kn, en, muc, kr=kr, etat=etat)
     if "shapes" in wl:
         ids = [system.add_shape(sh["kind"], radius=sh.get("radius", 0.0), L=sh.get("L"), verts=sh.get("verts"),
                                 faces=sh.get("faces"), crust=sh["crust"]) for sh in wl["shapes"]]
