@@ -84,6 +84,16 @@ void grains_gpu_destroy(grains_gpu_system* sys);
 int grains_gpu_set_contact_model(grains_gpu_system* sys, int matA, int matB, double kn, double en, double etat,
                                  double muc, double kr);
 
+/* HookeMemory contact model for a material pair (HookeMemoryContactForceModel,
+ * HookeMemoryContactForceModel.cpp:11-29, parameters :253-307):
+ * p = {kn, en, etat (-1: damping as the normal one), muc, kt, mur (0: no
+ * rolling friction), etarpf, epst}.  Each particle keeps, per partner met
+ * at the last evaluation of the time loop, the cumulative tangential spring,
+ * the previous normal and the rolling spring torque (at most 16 partners; a
+ * particle with more stops the run with GRAINS_DEVICE_ERROR).  Before
+ * grains_gpu_initialize. */
+int grains_gpu_set_memory_contact_model(grains_gpu_system* sys, int matA, int matB, const double* p);
+
 /* Adds n spheres in insertion order (ids continue from the current count).
  * radius, density, crust: n values each; material: n ints; x, v, w: 3n
  * values (w = space-fixed angular velocity); v and w may be NULL (zero).

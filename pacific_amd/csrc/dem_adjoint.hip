@@ -311,6 +311,8 @@ int grains_gpu_adjoint_forward(grains_gpu_system* s, int nsteps, int checkpoint_
   if (s->cfg.scheme != GRAINS_LEAPFROG) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates SecondOrderLeapFrog");
   if (s->cfg.precision != GRAINS_FP64) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates the FP64 step");
   if (s->has_shapes()) return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates systems of spheres");
+  if (s->has_memory())
+    return afail(s, GRAINS_CONFIG_ERROR, "the adjoint differentiates the Hooke model without contact memory");
   if (s->verlet)
     return afail(s, GRAINS_CONFIG_ERROR,
                  "the adjoint replays the per-step neighbour search: set the skin to 0 with "

@@ -90,6 +90,8 @@ __global__ __launch_bounds__(512) void k_batch(const BatchSys* __restrict__ sys,
   sp.pair_cap = 0;
   sp.scheme = kLeapFrog;
   sp.first_step = 0;
+  sp.stats = nullptr;
+  sp.mem = ContactMemory{};
   const BoxObstacle* bobs = obs + B.o0;
   const HookeModel* bmod = models + B.m0;
   const int nrounds = mode == 0 ? 1 : nsteps;
@@ -286,6 +288,7 @@ int grains_gpu_batch_create(grains_gpu_system* const* systems, int nsys, int pre
   for (int i = 0; i < nsys; ++i) {
     grains_gpu_system* s = systems[i];
     if (s->initialized) return bfail(b, GRAINS_ARGUMENT_ERROR, "batch members must not be initialised");
+    if (s->has_memory()) return bfail(b, GRAINS_CONFIG_ERROR, "batch members use contact models without memory");
     if (s->cfg.scheme != GRAINS_LEAPFROG) return bfail(b, GRAINS_CONFIG_ERROR, "batches run SecondOrderLeapFrog");
     if (s->has_shapes()) return bfail(b, GRAINS_CONFIG_ERROR, "batch members hold spheres only");
     // model checks and grid of the member (same rules as one system)

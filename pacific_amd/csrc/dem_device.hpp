@@ -318,6 +318,96 @@ DEM_HD bool hooke64(const HookeModel& m, d3 x0, d3 v0, d3 w0, double m0, double 
   return fn < ft;
 }
 
+// Quaternion::setRotFromTwoVectors (Quaternion.cpp:263-286): the rotation
+// taking u to v, (x, y, z, w) normalised.
+struct RotQ {
+  d3 v;
+  double w;
+};
+DEM_HD RotQ rot_from_two(const d3& u, const d3& v) {
+  double nn = sqrt(dot(u, u) * dot(v, v));
+  double re = nn + dot(u, v);
+  d3 vect;
+  if (re < 1.e-6 * nn) {
+    re = 0.;
+    vect = fabs(u.x) > fabs(u.z) ? d3{-u.y, u.x, 0.} : d3{0., -u.z, u.y};
+  } else {
+    vect = cross(u, v);
+  }
+  const double d = 1. / sqrt(vect.x * vect.x + vect.y * vect.y + vect.z * vect.z + re * re);
+  return RotQ{d * vect, d * re};
+}
+// Quaternion::rotateVector (Quaternion.cpp:293-298)
+DEM_HD d3 rotate_vec(const RotQ& q, const d3& x) {
+  const double nv = norm(q.v);
+  return ((q.w * q.w - nv * nv) * x + (2. * dot(x, q.v)) * q.v) + (2. * q.w) * cross(q.v, x);
+}
+
+// HookeMemoryContactForceModel (HookeMemoryContactForceModel.cpp:54-193),
+// p0 = self: normal spring-dashpot, tangential spring kt with memory kd
+// rotated into the current contact plane and capped at Coulomb, rolling
+// spring torque sr with memory (mur != 0).  existed: the pair was in contact
+// at the last evaluation (kd, pn, sr hold its memory; zero otherwise); they
+// are updated in place.  Returns the force on p0 and its rolling torque;
+// true when the Coulomb cap applied.
+DEM_HD bool hooke_memory64(const HookeModel& m, d3 x0, d3 v0, d3 w0, double m0, double req0, d3 x1, d3 v1, d3 w1,
+                           double m1, double req1, d3 P, d3 pen, double dt, bool existed, d3& kd, d3& pn, d3& sr,
+                           d3& f, d3& dM) {
+  d3 normal = pen;
+  normal /= norm(normal);
+  round_small(normal);
+  d3 tmpV = (v0 + cross(w0, P - x0)) - (v1 + cross(w1, P - x1));
+  d3 v_n = normal * dot(tmpV, normal);
+  d3 v_t = tmpV - v_n;
+  d3 delFN = m.kn * pen;
+  double avmass = 1. / (1. / m0 + 1. / m1);
+  double gamman = -2. * m.beta * sqrt(avmass * m.kn);
+  delFN -= gamman * v_n;
+  double normFN = norm(delFN);
+  double etat = m.etat == -1. ? -m.beta * sqrt(m.kn / avmass) : m.etat;
+  double gammat = 2. * etat * avmass;
+  d3 viscousFT = -gammat * v_t;
+  RotQ q{d3{0., 0., 0.}, 0.};
+  if (existed) {
+    q = rot_from_two(pn, normal);
+    kd = rotate_vec(q, kd);
+  }
+  kd += (dt * m.kt) * v_t;
+  pn = normal;
+  // computeTangentialVector (:54-65)
+  d3 tij{0., 0., 0.};
+  if (norm(v_t) > 0.001) {
+    tij = -v_t / norm(v_t);
+  } else {
+    const d3 nt = gammat * v_t, a = kd + nt;
+    if ((norm(kd) > m.epst || norm(nt) > m.epst) && norm(a) > m.epst) tij = -a / norm(a);
+  }
+  d3 delFT;
+  const bool slip = !(norm(-kd + viscousFT) <= m.muc * normFN);
+  if (!slip) {
+    delFT = norm(-kd + viscousFT) * tij;
+  } else {
+    if (m.kt != 0.) kd = ((-m.muc) * normFN * tij + viscousFT) / m.kt;
+    delFT = (m.muc * normFN) * tij;
+  }
+  dM = d3{0., 0., 0.};
+  if (m.mur != 0.) {
+    double Req = 1. / (1. / req0 + 1. / req1);
+    double kr = 3. * m.kn * m.mur * m.mur * Req * Req;
+    double etar = 3. * gamman * m.mur * m.mur * Req * Req;
+    double max_normMk = m.mur * Req * normFN;
+    d3 wrel = w0 - w1;
+    if (existed) sr = rotate_vec(q, sr);
+    sr -= (kr * dt) * wrel;
+    double normMk = norm(sr);
+    if (normMk > max_normMk) sr *= max_normMk / normMk;
+    dM = sr - (m.etarpf * etar) * wrel;
+  }
+  double coef = 1.;
+  f = coef * (delFN + delFT);
+  return slip;
+}
+
 // Same model in FP32 on lever arms relative to each body's centre.
 DEM_HD bool hooke32(const HookeModel& m, f3 l0, f3 v0, f3 w0, float m0, float req0, f3 l1, f3 v1, f3 w1,
                     float m1, float req1, f3 pen, f3& f, f3& dM) {
@@ -1016,7 +1106,9 @@ __device__ __forceinline__ void for_each_verlet(const Particles& P, const Sorted
 // Hooke-dashpot-Coulomb force and torque gathered on the particle
 // (HookeContactForceModel.cpp:121-163, Torsor.cpp:184-190); F enters holding
 // the weight.  MIXED evaluates the model in FP32 on lever arms.
-template <bool MIXED>
+// EXTRAS: contact memory models and statistics may occur (checked at run
+// time); without, the visitor carries none of their code.
+template <bool MIXED, bool EXTRAS = false>
 struct ForceVisitor {
   const Particles& P;
   const Sorted& S;
@@ -1029,6 +1121,52 @@ struct ForceVisitor {
   const Self& me;
   d3 F, T;
   f3 Fc, Tc;
+  int s = -1;     // storage index of self (contact memory)
+  int nmem = 0;   // memory entries written by this evaluation
+
+  // HookeMemory: this pair's entry of the last evaluation (zero when new),
+  // the force, and this evaluation's entry.  Evaluated in FP64 in both
+  // precisions.
+  __device__ __forceinline__ bool memory_pair(const HookeModel& hm, int pid2, d3 xo, d3 vo, d3 wo, double mo,
+                                              double reqo, d3 Pc, d3 ov) {
+    const ContactMemory& cm = sp.mem;
+    const size_t n = (size_t)sp.n;
+    const int c = cm.cur, o = c ^ 1;
+    int j = -1;
+    for (int i = 0, cnt = cm.n[c][s]; i < cnt; ++i)
+      if (cm.key[c][i * n + s] == pid2) {
+        j = i;
+        break;
+      }
+    d3 kd{0., 0., 0.}, pn{0., 0., 0.}, sr{0., 0., 0.};
+    if (j >= 0) {
+      const size_t e = j * n + s;
+      kd = d3{cm.kd[c][0][e], cm.kd[c][1][e], cm.kd[c][2][e]};
+      pn = d3{cm.pn[c][0][e], cm.pn[c][1][e], cm.pn[c][2][e]};
+      sr = d3{cm.sr[c][0][e], cm.sr[c][1][e], cm.sr[c][2][e]};
+    }
+    d3 f, dM;
+    const bool slip = hooke_memory64(hm, me.x, me.v, me.w, me.m, me.req, xo, vo, wo, mo, reqo, Pc, ov, sp.dt, j >= 0,
+                                     kd, pn, sr, f, dM);
+    const int jn = nmem++;
+    if (jn < cm.cap) {
+      const size_t e = jn * n + s;
+      cm.key[o][e] = pid2;
+      cm.kd[o][0][e] = kd.x;
+      cm.kd[o][1][e] = kd.y;
+      cm.kd[o][2][e] = kd.z;
+      cm.pn[o][0][e] = pn.x;
+      cm.pn[o][1][e] = pn.y;
+      cm.pn[o][2][e] = pn.z;
+      cm.sr[o][0][e] = sr.x;
+      cm.sr[o][1][e] = sr.y;
+      cm.sr[o][2][e] = sr.z;
+    }
+    F += f;
+    T += cross(Pc - me.x, f);
+    if (hm.mur != 0.) T += dM * 1.;
+    return slip;
+  }
 
   __device__ __forceinline__ void pair(int k2, int s2i, int pid2, d3 xo, double ro, d3 Pc, d3 ov, double d) {
     (void)k2;
@@ -1038,7 +1176,9 @@ struct ForceVisitor {
     d3 vo = ld3(P.v[cv], s2i), wo = ld3(P.w[cv], s2i);
     double mo = P.mass[s2i], reqo = P.req[s2i];
     bool slip;
-    if (MIXED) {
+    if (EXTRAS && hm.memory) {
+      slip = memory_pair(hm, pid2, xo, vo, wo, mo, reqo, Pc, ov);
+    } else if (MIXED) {
       f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - xo);
       f3 f, dM;
       slip = hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3of(vo), f3of(wo), (float)mo,
@@ -1063,7 +1203,9 @@ struct ForceVisitor {
     d3 cen{B.centre[0], B.centre[1], B.centre[2]};
     d3 zero{0., 0., 0.};
     bool slip;
-    if (MIXED) {
+    if (EXTRAS && hm.memory) {
+      slip = memory_pair(hm, B.id, cen, zero, zero, kObstacleMass, B.req, Pc, ov);
+    } else if (MIXED) {
       f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - cen);
       f3 f, dM;
       slip = hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3{0.f, 0.f, 0.f},
@@ -1082,7 +1224,7 @@ struct ForceVisitor {
   }
 };
 
-template <bool MIXED, bool SHAPES = false>
+template <bool MIXED, bool SHAPES = false, bool EXTRAS = false>
 __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted& S, const Grid& G,
                                                const BoxObstacle* __restrict__ obs, int nobs,
                                                const HookeModel* __restrict__ models, const StepParams& sp,
@@ -1091,20 +1233,27 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
                                                const ConvexGeo* mg = nullptr, int cx = 0,
                                                const NbrList* L = nullptr, const VerletState* VS = nullptr,
                                                int s = -1, const NbrList* CL = nullptr) {
-  ForceVisitor<MIXED> vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
+  using FV = ForceVisitor<MIXED, EXTRAS>;
+  FV vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
+  vis.s = s;
   if (VS && VS->stamp)
-    for_each_verlet<ForceVisitor<MIXED>, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx,
-                                                 CL && CL->count ? CL : nullptr);
+    for_each_verlet<FV, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx,
+                                CL && CL->count ? CL : nullptr);
   else if (L && L->count)
-    for_each_listed<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx);
+    for_each_listed<FV, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx);
   else
-    for_each_contact<ForceVisitor<MIXED>, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis,
-                                                  mg, cx);
+    for_each_contact<FV, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis, mg, cx);
   F = vis.F;
   T = vis.T;
   if (MIXED) {
     F += d3{(double)vis.Fc.x, (double)vis.Fc.y, (double)vis.Fc.z};
-    T = d3{(double)vis.Tc.x, (double)vis.Tc.y, (double)vis.Tc.z};
+    T += d3{(double)vis.Tc.x, (double)vis.Tc.y, (double)vis.Tc.z};
+  }
+  // this evaluation's contact memory of the particle (every particle, so
+  // that entries of contacts that ended are dropped)
+  if (EXTRAS && sp.mem.cap && s >= 0) {
+    sp.mem.n[sp.mem.cur ^ 1][s] = min(vis.nmem, sp.mem.cap);
+    if (vis.nmem > sp.mem.cap) raise_error(st, kMemoryOverflow, me.id, vis.nmem);
   }
 }
 

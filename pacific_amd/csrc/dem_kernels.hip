@@ -283,7 +283,7 @@ __device__ __forceinline__ void sphere_tail(const Particles& P, const Sorted& S,
   P.q[3][s] = kin.q.w;
 }
 
-template <int MODE, bool MIXED, bool SHAPES>
+template <int MODE, bool MIXED, bool SHAPES, bool EXTRAS>
 __device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, const Grid& G,
                                           const BoxObstacle* __restrict__ obs, const HookeModel* __restrict__ models,
                                           const StepParams& sp, DevStatus* st, int* pairs, const NbrList& L,
@@ -318,7 +318,7 @@ __device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, c
 
   ConvexGeo mg;
   if (SHAPES) mg = convex_of(P, s, me.x, cx);
-  sweep_particle<MIXED, SHAPES>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
+  sweep_particle<MIXED, SHAPES, EXTRAS>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
                                 cx, &L, &VS, s, &CL);
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
@@ -389,21 +389,23 @@ __device__ __forceinline__ void block_stats(const StatAcc& a, double* out) {
 // read and written in contiguous runs; its sorted slot locates the cell data.
 // Non-spherical bodies: one thread per sorted slot, so that a wave holds
 // neighbouring bodies (GJK dominates, and neighbours share branches).
-// STATS: contact statistics may be collected (sp.stats set).  Spheres have a
-// separate instantiation without them, so that the common launch carries
-// none of their cost; the GJK kernels, heavy to compile, check at run time.
-template <int MODE, bool MIXED, bool SHAPES, bool STATS>
+// EXTRAS: contact statistics (sp.stats set) and contact memory models may
+// occur, checked at run time.  Spheres have a separate instantiation
+// without them, so that the common launch carries none of their cost; the
+// GJK kernels, heavy to compile, always check at run time.
+template <int MODE, bool MIXED, bool SHAPES, bool EXTRAS>
 __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs,
                                                 const HookeModel* __restrict__ models, StepParams sp, DevStatus* st,
                                                 int* pairs, NbrList L, VerletState VS, NbrList CL) {
   const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (!STATS) {
-    if (t < sp.n) force_one<MODE, MIXED, SHAPES>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, nullptr);
+  if (!EXTRAS) {
+    if (t < sp.n) force_one<MODE, MIXED, SHAPES, false>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, nullptr);
     return;
   }
   StatAcc acc;
   if (t < sp.n)
-    force_one<MODE, MIXED, SHAPES>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, sp.stats ? &acc : nullptr);
+    force_one<MODE, MIXED, SHAPES, true>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t,
+                                         sp.stats ? &acc : nullptr);
   if (sp.stats) block_stats(acc, sp.stats + 5 * (size_t)blockIdx.x);
 }
 
@@ -602,7 +604,7 @@ static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, 
     hipLaunchKernelGGL((k_forces<MODE, MIXED, true, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
                        pairs, L, V, CL);
   } else {
-    if (sp.stats)
+    if (sp.stats || sp.mem.cap)
       hipLaunchKernelGGL((k_forces<MODE, MIXED, false, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
                          pairs, L, V, CL);
     else

@@ -126,6 +126,22 @@ int grains_gpu_system::allocate() {
   HIPCHECK(this, track(this, &d_status, 1));
   if (verlet)
     for (int a = 0; a < 3; ++a) HIPCHECK(this, track(this, &V.xb[a], n));
+  mem = dem::ContactMemory{};
+  if (has_memory() && n > 0) {
+    const size_t m = (size_t)kMemoryCap * n;
+    for (int b = 0; b < 2; ++b) {
+      HIPCHECK(this, track(this, &mem.n[b], n));
+      HIPCHECK(this, hipMemsetAsync(mem.n[b], 0, sizeof(int) * n, stream));
+      HIPCHECK(this, track(this, &mem.key[b], m));
+      for (int a = 0; a < 3; ++a) {
+        HIPCHECK(this, track(this, &mem.kd[b][a], m));
+        HIPCHECK(this, track(this, &mem.pn[b][a], m));
+        HIPCHECK(this, track(this, &mem.sr[b][a], m));
+      }
+    }
+    mem.cap = kMemoryCap;
+    mem.cur = 0;
+  }
   HIPCHECK(this, track(this, &d_stat_part, 5 * ((n + 255) / 256) + 5));
   HIPCHECK(this, track(this, &d_stat_tot, 8));
   HIPCHECK(this, hipMemsetAsync(d_stat_tot, 0, 8 * sizeof(double), stream));
@@ -296,6 +312,7 @@ StepParams grains_gpu_system::params() const {
   sp.first_step = 0;
   // the adjoint's replays are not counted again
   sp.stats = stats_on && !recording ? d_stat_part : nullptr;
+  sp.mem = mem;
   return sp;
 }
 
@@ -338,11 +355,19 @@ int grains_gpu_system::bin_launch() {
   return GRAINS_OK;
 }
 
-int grains_gpu_system::forces(int mode, int first_step) {
+bool grains_gpu_system::has_memory() const {
+  if (has_default && default_model.memory) return true;
+  for (const auto& m : models)
+    if (m.defined && m.memory) return true;
+  return false;
+}
+
+int grains_gpu_system::forces(int mode, int first_step, bool commit) {
   int e = reset_pair_counter();
   if (e) return e;
   StepParams sp = params();
   sp.first_step = first_step;
+  if (!commit) sp.stats = nullptr;
   const dem::NbrList L = nbr_list();
   if (L.count && !verlet) {
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
@@ -361,6 +386,8 @@ int grains_gpu_system::forces(int mode, int first_step) {
                                d_pairs, L, vstate(next_stamp()), CL, stream));
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
   if (sp.stats) HIPCHECK(this, launch_stats_fold(d_stat_part, nparticles, d_stat_tot, force_time, stream));
+  // the evaluation wrote the other memory buffer: it becomes current
+  if (commit && mem.cap) mem.cur ^= 1;
   if (mode == kModeForcesKick) cur ^= 2;
   if (mode == kModeFused) cur ^= 3;
   return GRAINS_OK;
@@ -428,6 +455,11 @@ int grains_gpu_system::check_status() {
   if (st.code) {
     static const char* names[] = {"", "ContactError", "MotionError", "SimulationError"};
     char buf[160];
+    if (st.code == kMemoryOverflow) {
+      snprintf(buf, sizeof(buf), "particle %d has %d contacts with memory, more than the %d kept per particle", st.id0,
+               st.id1, kMemoryCap);
+      return fail(this, GRAINS_DEVICE_ERROR, buf);
+    }
     snprintf(buf, sizeof(buf), "%s raised on the device (bodies %d and %d)", names[st.code & 3], st.id0, st.id1);
     return fail(this, st.code, buf);
   }
@@ -491,16 +523,7 @@ int grains_gpu_create(const grains_gpu_config* cfg, grains_gpu_system** out) {
 
 void grains_gpu_destroy(grains_gpu_system* sys) { delete sys; }
 
-int grains_gpu_set_contact_model(grains_gpu_system* s, int matA, int matB, double kn, double en, double etat,
-                                 double muc, double kr) {
-  if (!s) return GRAINS_ARGUMENT_ERROR;
-  if (matA >= kMaxMaterials || matB >= kMaxMaterials) return fail(s, GRAINS_ARGUMENT_ERROR, "material index too large");
-  if (!(en > 0.) || en > 1.) return fail(s, GRAINS_CONFIG_ERROR, "en must lie in (0, 1]");
-  HookeModel m;
-  m.kn = kn; m.en = en; m.etat = etat; m.muc = muc; m.kr = kr;
-  m.beta = hooke_beta(en);
-  m.defined = 1;
-  m.pad = 0;
+static int set_model(grains_gpu_system* s, int matA, int matB, const HookeModel& m) {
   if (matA < 0 || matB < 0) {
     s->default_model = m;
     s->has_default = true;
@@ -512,6 +535,34 @@ int grains_gpu_set_contact_model(grains_gpu_system* s, int matA, int matB, doubl
     HIPCHECK(s, hipMemcpy(s->d_models, s->models, sizeof(s->models), hipMemcpyHostToDevice));
   return GRAINS_OK;
 }
+
+int grains_gpu_set_contact_model(grains_gpu_system* s, int matA, int matB, double kn, double en, double etat,
+                                 double muc, double kr) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  if (matA >= kMaxMaterials || matB >= kMaxMaterials) return fail(s, GRAINS_ARGUMENT_ERROR, "material index too large");
+  if (!(en > 0.) || en > 1.) return fail(s, GRAINS_CONFIG_ERROR, "en must lie in (0, 1]");
+  HookeModel m{};
+  m.kn = kn; m.en = en; m.etat = etat; m.muc = muc; m.kr = kr;
+  m.beta = hooke_beta(en);
+  m.defined = 1;
+  m.memory = 0;
+  return set_model(s, matA, matB, m);
+}
+
+int grains_gpu_set_memory_contact_model(grains_gpu_system* s, int matA, int matB, const double* p) {
+  if (!s || !p) return GRAINS_ARGUMENT_ERROR;
+  if (matA >= kMaxMaterials || matB >= kMaxMaterials) return fail(s, GRAINS_ARGUMENT_ERROR, "material index too large");
+  if (!(p[1] > 0.) || p[1] > 1.) return fail(s, GRAINS_CONFIG_ERROR, "en must lie in (0, 1]");
+  if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "a model with memory is defined before initialisation");
+  HookeModel m{};
+  m.kn = p[0]; m.en = p[1]; m.etat = p[2]; m.muc = p[3]; m.kr = 0.;
+  m.kt = p[4]; m.mur = p[5]; m.etarpf = p[6]; m.epst = p[7];
+  m.beta = hooke_beta(m.en);
+  m.defined = 1;
+  m.memory = 1;
+  return set_model(s, matA, matB, m);
+}
+
 
 int grains_gpu_add_spheres(grains_gpu_system* s, size_t n, const double* radius, const double* density,
                            const double* crust, const int* material, const double* x, const double* v,
@@ -836,11 +887,9 @@ int grains_gpu_synchronize(grains_gpu_system* s) {
 int grains_gpu_compute_forces(grains_gpu_system* s) {
   if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
   HIPCHECK(s, hipSetDevice(s->device));
-  // an evaluation outside the time loop does not enter the statistics
-  const bool on = s->stats_on;
-  s->stats_on = false;
-  int e = s->forces(kModeForcesOnly, 0);
-  s->stats_on = on;
+  // an evaluation outside the time loop neither enters the statistics nor
+  // advances the contact memory
+  int e = s->forces(kModeForcesOnly, 0, false);
   if (e) return e;
   return s->check_status();
 }
@@ -958,7 +1007,7 @@ int grains_gpu_set_state(grains_gpu_system* s, const double* x, const double* v,
   int e = s->bin_current();
   if (e) return e;
   if (recompute_forces) {
-    if ((e = s->forces(kModeForcesOnly, 0))) return e;
+    if ((e = s->forces(kModeForcesOnly, 0, false))) return e;
   }
   return s->check_status();
 }

@@ -89,6 +89,10 @@ struct grains_gpu_system {
   int stamp = 0;
   // contact statistics (AppCollision): per-block partials of the last force
   // launch and running totals; force_time = time of the next force launch
+  // HookeMemory contact memory (allocated when a model has memory)
+  static constexpr int kMemoryCap = 16;
+  dem::ContactMemory mem{};
+  bool has_memory() const;
   bool stats_on = false;
   double* d_stat_part = nullptr;
   double* d_stat_tot = nullptr;
@@ -112,7 +116,8 @@ struct grains_gpu_system {
   dem::StepParams params() const;
   int bin_current();
   int reset_pair_counter();
-  int forces(int mode, int first_step);
+  // commit: the evaluation belongs to the time loop (contact memory advances)
+  int forces(int mode, int first_step, bool commit = true);
   // optional per-launch HIP events (force sweeps and binning sequences)
   struct Profile {
     std::vector<hipEvent_t> force_ev, bin_ev, detect_ev;

@@ -13,7 +13,24 @@ constexpr int kMaxMaterials = 16;
 struct HookeModel {
   double kn, en, etat, muc, kr, beta;
   int defined;
-  int pad;
+  int memory;  // HookeMemory: tangential spring kt, rolling friction mur (on when != 0)
+  double kt, mur, etarpf, epst;
+};
+
+// Contact memory of the HookeMemory model (Component::m_contactMap,
+// Component.cpp:634-690): per particle, one entry per partner met by the
+// last committed force evaluation -- partner id (obstacles negative),
+// cumulative kt * tangential motion, previous normal, cumulative rolling
+// spring torque.  Entry j of particle s at [j * N + s]; buffer cur holds the
+// last evaluation's entries, the next evaluation writes the other one.
+struct ContactMemory {
+  int* n[2];
+  int* key[2];
+  double* kd[2][3];
+  double* pn[2][3];
+  double* sr[2][3];
+  int cap;
+  int cur;
 };
 
 // Static box obstacle (Box.cpp:27-125, Transform.cpp:71-77, Convex.cpp:29-41)
@@ -147,7 +164,7 @@ struct DevStatus {
   int n_rebuild;     // Verlet list builds
 };
 
-enum ErrCode { kOk = 0, kContactError = 1, kMotionError = 2, kSimulationError = 3 };
+enum ErrCode { kOk = 0, kContactError = 1, kMotionError = 2, kSimulationError = 3, kMemoryOverflow = 4 };
 enum Scheme { kLeapFrog = 0, kFirstOrderExplicit = 1, kSecondOrderExplicit = 2, kAdamsBashforth2 = 3 };
 
 struct StepParams {
@@ -162,6 +179,7 @@ struct StepParams {
   int scheme;
   int first_step;     // Adams-Bashforth first step (Euler start)
   double* stats;      // contact statistics: 5 partial sums per force block (nullptr: not collected)
+  ContactMemory mem;  // HookeMemory tables (mem.cap == 0: none)
 };
 
 }  // namespace dem

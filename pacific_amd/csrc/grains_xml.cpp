@@ -385,6 +385,8 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
   struct ModelDef {
     int a, b;
     double kn, en, etat, muc, kr;
+    bool memory = false;  // <HookeMemory>: kt, mur, etarpf, epst
+    double kt = 0., mur = 0., etarpf = 0., epst = 0.;
   };
   std::vector<ModelDef> models;
   std::vector<double> positions;
@@ -512,15 +514,27 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
         }
         for (const XNode& t : k.kids) {
           if (t.name == "Material") continue;
-          if (t.name == "Hooke") law = &t;
-          else c.bad("contact force model <" + t.name + "> is not supported (only <Hooke>)");
+          if (t.name == "Hooke" || t.name == "HookeMemory") law = &t;
+          else c.bad("contact force model <" + t.name + "> is not supported (only <Hooke> and <HookeMemory>)");
         }
         if (!law) { c.bad("no contact force model in <" + k.name + ">"); break; }
         md.kn = dvalue(law->child("kn"), c, "kn");
         md.en = dvalue(law->child("en"), c, "en");
         md.muc = dvalue(law->child("muc"), c, "muc");
-        md.kr = dvalue(law->child("kr"), c, "kr");
         md.etat = law->child("etat") ? dvalue(law->child("etat"), c, "etat") : -1.;
+        if (law->name == "HookeMemory") {
+          // HookeMemoryContactForceModel::defineParameters (:253-307); a
+          // missing <epst> leaves the threshold at 0 there (the default is
+          // stored under another key)
+          md.memory = true;
+          md.kr = 0.;
+          md.kt = dvalue(law->child("kt"), c, "kt");
+          md.mur = law->child("mur") ? dvalue(law->child("mur"), c, "mur") : 0.;
+          md.etarpf = law->child("etarpf") ? dvalue(law->child("etarpf"), c, "etarpf") : 0.;
+          md.epst = law->child("epst") ? dvalue(law->child("epst"), c, "epst") : 0.;
+        } else {
+          md.kr = dvalue(law->child("kr"), c, "kr");
+        }
         models.push_back(md);
       }
     }
@@ -661,7 +675,12 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
   s->coupled_ndt = coupled_ndt;
   int e;
   for (auto& m : models) {
-    e = grains_gpu_set_contact_model(s, m.a, m.b, m.kn, m.en, m.etat, m.muc, m.kr);
+    if (m.memory) {
+      const double p[8] = {m.kn, m.en, m.etat, m.muc, m.kt, m.mur, m.etarpf, m.epst};
+      e = grains_gpu_set_memory_contact_model(s, m.a, m.b, p);
+    } else {
+      e = grains_gpu_set_contact_model(s, m.a, m.b, m.kn, m.en, m.etat, m.muc, m.kr);
+    }
     if (e) return e;
   }
   // insertion at the initial time, class by class in file order

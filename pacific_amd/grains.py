@@ -47,6 +47,7 @@ EXPORTS = [
     "grains_gpu_to_fluid", "grains_gpu_set_velocities", "grains_gpu_count_rigid_bodies",
     "grains_gpu_time_interval", "grains_gpu_closest_point", "grains_gpu_set_neighbor_list",
     "grains_gpu_neighbor_stats", "grains_gpu_set_contact_stats", "grains_gpu_contact_stats",
+    "grains_gpu_set_memory_contact_model",
 ]
 
 
@@ -80,6 +81,7 @@ def lib():
         L.grains_gpu_set_neighbor_list.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.grains_gpu_neighbor_stats.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
         L.grains_gpu_set_contact_stats.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_set_memory_contact_model.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.grains_gpu_contact_stats.argtypes = [vp, dp]
         L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
         L.grains_gpu_num_particles.argtypes = [vp]
@@ -261,6 +263,11 @@ class GrainsGPU:
 
     def contact_model(self, matA, matB, kn, en, muc, kr=0.0, etat=-1.0):
         self._check(lib().grains_gpu_set_contact_model(self.h, matA, matB, kn, en, etat, muc, kr))
+
+    def memory_contact_model(self, matA, matB, kn, en, muc, kt, etat=-1.0, mur=0.0, etarpf=0.0, epst=0.0):
+        """HookeMemory contact model (tangential spring kt, rolling friction mur)."""
+        p = np.array([kn, en, etat, muc, kt, mur, etarpf, epst], np.float64)
+        self._check(lib().grains_gpu_set_memory_contact_model(self.h, int(matA), int(matB), _d(p)))
 
     def add_spheres(self, radius, density, crust, material, x, v=None, w=None):
         x = _c64(x, (-1, 3))

@@ -1,0 +1,148 @@
+"""HookeMemory contact model on the device
+(HookeMemoryContactForceModel.cpp; grains_gpu_set_memory_contact_model).
+
+* against the oracle, which keeps the reference's contact maps (one
+  evaluation per pair, the partner's entry set to the mirror image): dense
+  packings with walls and an inclined plane, with the Verlet and the
+  per-step lists, in FP64 -- the same contact records, forces and torques
+  within 1e-9 of the contact force scale (each side of a pair evaluates its
+  own contact point, which agrees with the partner's to rounding, so the
+  springs drift apart at rounding level) and positions within 1e-12 m;
+* rigid-body closed forms as for the oracle: rolling without slipping at
+  5/7 g sin(theta), a rolling spring holding a sphere at rest;
+* a particle meeting more partners than its memory keeps stops the run
+  with an error naming it;
+* <HookeMemory> in an input file.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from pacific_amd import GrainsError, GrainsGPU, MIXED, workloads
+from tests.helpers import contact_scale, make_gpu, make_oracle, pair_set, rel_force_error
+
+pytestmark = pytest.mark.gpu
+
+MEM = ("memory", 0, 0, 1e4, 0.6, 0.4, 2.0 / 7.0 * 1e4, -1.0, 0.1, 1.0, 0.0)
+RHO, G = 2500.0, 9.81
+
+
+def with_memory(wl, model=MEM):
+    wl["models"] = [model]
+    return wl
+
+
+CASES = {
+    "dense_walls": (lambda: with_memory(workloads.dense_pack(side=6, sigma=0.2)), 300),
+    "inclined": (lambda: with_memory(workloads.inclined_plane(3000)), 600),
+}
+
+
+@pytest.mark.parametrize("skin", [None, 0.0])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_memory_model_matches_oracle(gpu, case, skin):
+    make, n = CASES[case]
+    wl = make()
+    o = make_oracle(wl)
+    g = make_gpu(wl, skin=skin)
+    npairs = 0
+    for _ in range(3):
+        o.step(n // 3)
+        g.step(n // 3)
+        so, sg = o.state(), g.state()
+        assert pair_set(g.pairs()) == pair_set(o.pairs())
+        npairs += len(pair_set(o.pairs()))
+        _, _, wgt = o.masses()
+        scale = contact_scale(so, wgt)
+        assert rel_force_error(sg["F"], so["F"], scale) <= 1e-9
+        assert rel_force_error(sg["T"], so["T"], scale * 1e-3) <= 1e-9
+        assert np.abs(sg["x"] - so["x"]).max() <= 1e-12
+    assert npairs > 40
+
+
+def test_memory_model_mixed_precision_runs_close(gpu):
+    make, n = CASES["dense_walls"]
+    wl = make()
+    o = make_oracle(wl)
+    g = make_gpu(wl, precision=MIXED)
+    o.step(n)
+    g.step(n)
+    so, sg = o.state(), g.state()
+    assert pair_set(g.pairs()) == pair_set(o.pairs())
+    _, _, wgt = o.masses()
+    assert rel_force_error(sg["F"], so["F"], contact_scale(so, wgt)) <= 1e-5
+
+
+def slope(theta, model, sphere=True, r=1e-3):
+    R = workloads._rot_y(theta)
+    n = R[:, 2]
+    m = RHO * 4 / 3 * math.pi * r ** 3
+    delta = m * G * math.cos(math.radians(theta)) / model[3]
+    wl = {
+        "origin": (-0.03, -0.03, -0.03), "max": (0.03, 0.03, 0.03), "periodic": (0, 0, 0), "cellsize_factor": 1.0,
+        "gravity": (0.0, 0.0, -G), "dt": 2e-6,
+        "radius": np.array([r]), "density": np.array([RHO]), "crust": np.array([5e-5]),
+        "material": np.zeros(1, np.int32), "x": np.array([(0.001 + r - delta) * n]), "v": np.zeros((1, 3)),
+        "w": np.zeros((1, 3)), "boxes": [((0.05, 0.05, 0.002), (0.0, 0.0, 0.0), R.ravel(), 5e-5, 0)],
+        "models": [model],
+    }
+    e1 = R[:, 0]
+    return wl, R, (e1 if e1[2] < 0 else -e1)
+
+
+def test_rolling_without_slipping_on_device(gpu):
+    theta, r = 20.0, 1e-3
+    wl, R, d = slope(theta, ("memory", 0, 0, 1e4, 0.3, 0.5, 2.0 / 7.0 * 1e4, -1.0, 0.0, 0.0, 0.0))
+    g = make_gpu(wl, record=False)
+    t, v = [], []
+    for k in range(40):
+        g.step(500)
+        t.append((k + 1) * 500 * 2e-6)
+        v.append(float(g.state()["v"][0] @ d))
+    a = np.polyfit(np.array(t[10:]), np.array(v[10:]), 1)[0]
+    assert a == pytest.approx(5.0 / 7.0 * G * math.sin(math.radians(theta)), rel=1e-2)
+    s = g.state()
+    assert float(s["w"][0] @ np.cross(R[:, 2], d)) * r == pytest.approx(float(s["v"][0] @ d), rel=1e-2)
+
+
+def test_rolling_spring_holds_sphere_on_device(gpu):
+    wl, R, d = slope(15.0, ("memory", 0, 0, 1e4, 0.3, 0.5, 2.0 / 7.0 * 1e4, -1.0, 0.5, 1.0, 0.0))
+    g = make_gpu(wl, record=False)
+    g.step(20000)
+    s = g.state()
+    assert abs(float(s["v"][0] @ d)) < 1e-4
+    m = RHO * 4 / 3 * math.pi * 1e-9
+    assert np.abs(s["F"][0]).max() < 1e-3 * m * G
+
+
+def test_memory_capacity_error(gpu):
+    """A sphere of radius 3 mm touched by 30 spheres of 0.5 mm."""
+    R0, r, k = 3e-3, 0.5e-3, 30
+    i = np.arange(k) + 0.5
+    phi, th = np.arccos(1 - 2 * i / k), np.pi * (1 + 5 ** 0.5) * i
+    u = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], axis=1)
+    c = np.array([0.01, 0.01, 0.01])
+    x = np.vstack([c, c + (R0 + r - 2e-6) * u])
+    g = GrainsGPU((0, 0, 0), (0.02, 0.02, 0.02), gravity=(0, 0, 0), dt=1e-6)
+    g.memory_contact_model(0, 0, 1e4, 0.6, 0.4, 2857.0)
+    g.add_spheres(np.r_[R0, np.full(k, r)], RHO, 5e-5, 0, x)
+    with pytest.raises(GrainsError) as e:
+        g.init()
+        g.step(2)
+    assert "more than the 16 kept per particle" in str(e.value)
+
+
+def test_xml_hooke_memory(gpu, tmp_path):
+    from tests.test_xml import write_case
+    path, _ = write_case(tmp_path, n=300, **{
+        "<Hooke>\n          <kn>1.e4</kn>": "<HookeMemory>\n          <kt>2857.</kt>\n          <kn>1.e4</kn>",
+        "</kr>\n        </Hooke>": "</kr>\n        </HookeMemory>",
+        "<Hooke><kn>1.e4</kn>": "<HookeMemory><kt>2857.</kt><kn>1.e4</kn>",
+        "<etat>-1</etat></Hooke>": "<etat>-1</etat></HookeMemory>",
+    })
+    text = open(path).read()
+    assert text.count("<HookeMemory>") == 2 and "<Hooke>" not in text
+    g = GrainsGPU.from_xml(path)
+    g.step(500)
+    assert g.n == 300

@@ -113,7 +113,7 @@ def test_valid_file_parses(tmp_path):
 
 
 @pytest.mark.parametrize("sub,frag", [
-    (("<Hooke>", "<HookeMemory>", "</Hooke>", "</HookeMemory>"), "not supported"),
+    (("<Hooke>", "<HookeMassless>", "</Hooke>", "</HookeMassless>"), "not supported"),
     (("<Sphere Radius=\"0.001\"/>", "<Cylinder Radius=\"0.001\" Height=\"0.002\"/>"), "other than <Sphere>, <Box> and <Polyhedron>"),
     (("<Mode Type=\"InitialTime\"/>", "<Mode Type=\"OverTime\"/>"), "InitialTime"),
     (("<Gravity GX=\"0.\" GY=\"0.\" GZ=\"-9.81\"/>", ""), "Gravity"),
