@@ -8,6 +8,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -47,7 +49,16 @@ def _run(cmd):
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    out = _run([hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + srcs)
+    # one compiler process per source (the kernel files take minutes each),
+    # objects in a scratch folder outside the tree, then one link
+    with tempfile.TemporaryDirectory(prefix="grains_gpu_build_") as bdir:
+        objs = [os.path.join(bdir, os.path.basename(src) + ".o") for src in srcs]
+        jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            outs = list(pool.map(lambda so: _run([hipcc()] + HIPCC_FLAGS + ["-c", "-o", so[1], so[0]]),
+                                 zip(srcs, objs)))
+        out = "\n".join(o for o in outs if o.strip())
+        out += _run([hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     cli_src = os.path.join(CSRC, "grains_cli.cpp")
     if os.path.exists(cli_src):
