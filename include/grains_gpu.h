@@ -222,6 +222,50 @@ int grains_gpu_contact_stats(grains_gpu_system* sys, double* out);
 int grains_gpu_set_record_pairs(grains_gpu_system* sys, int on);
 int grains_gpu_get_pairs(grains_gpu_system* sys, int* buf, size_t cap, size_t* n);
 
+/* ---- Output of the time loop (host side, over the entry points above) ---- */
+
+/* Particle class of the particles added next (default 0): the index of
+ * their <Particle> entry, written as the RawData particle type
+ * (Particle::getGeometricType, Particle.cpp:1651-1654; set by
+ * AllComponents.cpp:955).  Before grains_gpu_initialize. */
+int grains_gpu_set_particle_class(grains_gpu_system* sys, int cls);
+
+/* Maximum and mean translational speed of the particles
+ * (AllComponents::ComputeMaxMeanVelocity, AllComponents.cpp:1781-1821;
+ * obstacles do not move here). */
+int grains_gpu_velocity_max_mean(grains_gpu_system* sys, double* vmax, double* vmean);
+
+/* Coordination numbers in id order: the number of distinct components
+ * (particles and obstacles) each particle was in contact with at the last
+ * force evaluation (Particle::addContactingComponentID, Particle.cpp:
+ * 1702-1706, reset by InitializeForce :1408-1415).  Needs contact records
+ * (grains_gpu_set_record_pairs) during that evaluation. */
+int grains_gpu_coordination_numbers(grains_gpu_system* sys, int* out);
+
+/* One RawData frame at the current time (RawDataPostProcessingWriter::
+ * one_output_Standard, RawDataPostProcessingWriter.cpp:362-600): the files
+ * <root>_position_{x,y,z}.dat, _translational_velocity_{x,y,z}.dat,
+ * _angular_velocity_{x,y,z}.dat, _coordinationNumber.dat and
+ * _particleType.dat (last frame only); text (scientific, 6 decimals) or
+ * binary.  truncate != 0 starts the files (ReloadType "new"), else the
+ * frame is appended.  coord: coordination numbers in id order, or NULL
+ * for zeros. */
+int grains_gpu_write_rawdata(grains_gpu_system* sys, const char* root, int binary, int truncate, const int* coord);
+
+/* Output settings of grains_gpu_run (the XML front-end sets them from
+ * <TimeSave Start End Every>, <RestartFile Name> and <RawData Name
+ * WritingMode>): output times, root of <root>_VelocityMaxMean.dat (NULL or
+ * "": none), RawData root (NULL or "": none). */
+int grains_gpu_set_outputs(grains_gpu_system* sys, const double* save_times, size_t nsave, const char* save_root,
+                           const char* rawdata_root, int rawdata_binary);
+
+/* Grains::Simulation to <TimeInterval End> with its outputs (Grains.cpp:
+ * 279-452 and the initial and final outputs :130-146, :199-211): at the
+ * start, at every output time and at the end a line "t  max|v|  mean|v|" in
+ * <save_root>_VelocityMaxMean.dat; at the start and at every output time a
+ * RawData frame.  Restart files are not written. */
+int grains_gpu_run(grains_gpu_system* sys, int verbose);
+
 size_t grains_gpu_num_particles(const grains_gpu_system* sys);
 /* <TimeInterval Start End> and <TimeStep dt> of the system */
 int grains_gpu_time_interval(const grains_gpu_system* sys, double* t_start, double* t_end, double* dt);

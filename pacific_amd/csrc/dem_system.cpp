@@ -609,6 +609,7 @@ int grains_gpu_add_spheres(grains_gpu_system* s, size_t n, const double* radius,
     s->h_q.push_back(0.);
     s->h_q.push_back(1.);
     s->h_shape.push_back(-1);
+    s->h_class.push_back(s->next_class);
     for (int j = 0; j < 9; ++j) s->h_B.push_back(j % 4 == 0 ? 1. : 0.);
     for (int a = 0; a < 3; ++a) s->h_I3.push_back(I);
   }
@@ -691,6 +692,7 @@ int grains_gpu_add_particles(grains_gpu_system* s, int shape, size_t n, const do
     s->h_wb.push_back(wb.z);
     for (int a = 0; a < 4; ++a) s->h_q.push_back(q[a]);
     s->h_shape.push_back(shape);
+    s->h_class.push_back(s->next_class);
     for (int j = 0; j < 9; ++j) s->h_B.push_back(Bm[j]);
     s->h_I3.push_back(I[0]);
     s->h_I3.push_back(I[3]);

@@ -18,6 +18,15 @@ struct grains_gpu_system {
   int nparticles = 0;
   double time = 0.;
   double t_end = 0.;  // <TimeInterval End> of an XML-built system
+  // outputs of the time loop: <TimeSave> times, <RestartFile Name> (root of
+  // _VelocityMaxMean.dat), <RawData Name WritingMode>
+  std::vector<double> save_times;
+  std::string save_root, rawdata_root;
+  bool rawdata_binary = false;
+  // particle class of each particle (index of its <Particle> in the input;
+  // the RawData particle type), id order, and the class of the next ones
+  std::vector<int> h_class;
+  int next_class = 0;
   long steps_done = 0;
   bool initialized = false;
   int cur = 0;  // bit 0: position buffer, bit 1: velocity buffer

@@ -395,6 +395,9 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
   double amp_t = 0., amp_r = 0.;
   unsigned seed = 0;
   double t_end = 0.;
+  std::vector<double> save_times;
+  std::string save_root, rawdata_root;
+  bool rawdata_binary = false;
   double coupled_dt[2] = {0., 0.};
   size_t coupled_ndt = 0;
 
@@ -573,6 +576,24 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
         t_end = dattr(ti, "End", c);
         cfg.dt = dattr(sim->child("TimeStep"), "dt", c);
       }
+      // outputs: <TimeSave Start End Every> (Grains.cpp:1224-1235), the
+      // <RestartFile Name> root (:1174-1182), RawData writers
+      // (PostProcessingWriterBuilderFactory.cpp:14-20, RawDataPostProcessingWriter.cpp:20-30)
+      if (const XNode* tsv = sim->child("TimeSave")) {
+        const double t0 = dattr(tsv, "Start", c), t1 = dattr(tsv, "End", c);
+        double every = dattr(tsv, "Every", c);
+        if (every < cfg.dt) every = cfg.dt;
+        if (c.ok && every > 0.)
+          for (double t = t0; t - t1 < 0.01 * cfg.dt; t += every) save_times.push_back(t);
+      }
+      if (const XNode* rf = sim->child("RestartFile")) save_root = rf->attr("Name");
+      if (const XNode* pp = sim->child("PostProcessing"))
+        if (const XNode* wr = pp->child("Writers"))
+          if (const XNode* rd = wr->child("RawData")) {
+            rawdata_root = rd->attr("Name");
+            rawdata_binary = rd->attr("WritingMode") == "Binary";
+            if (rawdata_root.empty()) c.bad("<RawData> needs a Name");
+          }
       if (const XNode* tint = sim->child("TimeIntegration")) {
         std::string t = tint->attr("Type");
         if (t == "SecondOrderLeapFrog") cfg.scheme = GRAINS_LEAPFROG;
@@ -668,6 +689,10 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
     return GRAINS_CONFIG_ERROR;
   }
   s->t_end = t_end;
+  s->save_times = save_times;
+  s->save_root = save_root;
+  s->rawdata_root = rawdata_root;
+  s->rawdata_binary = rawdata_binary;
   s->track_basis = opt.track_basis;
   s->fluid_corrected = opt.fluid_corrected;
   s->coupled_min_dt = coupled_dt[0];
@@ -703,6 +728,8 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
   srand(seed);
   size_t k = 0;
   for (auto& sc : classes) {
+    // RawData particle type: the class's index among <Particle> entries
+    s->next_class = (int)(&sc - &classes[0]);
     for (int i = 0; i < sc.number; ++i, ++k) {
       double v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
       if (vmode == 1) {

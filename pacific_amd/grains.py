@@ -47,7 +47,8 @@ EXPORTS = [
     "grains_gpu_to_fluid", "grains_gpu_set_velocities", "grains_gpu_count_rigid_bodies",
     "grains_gpu_time_interval", "grains_gpu_closest_point", "grains_gpu_set_neighbor_list",
     "grains_gpu_neighbor_stats", "grains_gpu_set_contact_stats", "grains_gpu_contact_stats",
-    "grains_gpu_set_memory_contact_model",
+    "grains_gpu_set_memory_contact_model", "grains_gpu_set_particle_class", "grains_gpu_velocity_max_mean",
+    "grains_gpu_coordination_numbers", "grains_gpu_write_rawdata", "grains_gpu_set_outputs", "grains_gpu_run",
 ]
 
 
@@ -83,6 +84,12 @@ def lib():
         L.grains_gpu_set_contact_stats.argtypes = [vp, ctypes.c_int]
         L.grains_gpu_set_memory_contact_model.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.grains_gpu_contact_stats.argtypes = [vp, dp]
+        L.grains_gpu_set_particle_class.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_velocity_max_mean.argtypes = [vp, dp, dp]
+        L.grains_gpu_coordination_numbers.argtypes = [vp, ip]
+        L.grains_gpu_write_rawdata.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ip]
+        L.grains_gpu_set_outputs.argtypes = [vp, dp, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.grains_gpu_run.argtypes = [vp, ctypes.c_int]
         L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
         L.grains_gpu_num_particles.argtypes = [vp]
         L.grains_gpu_num_particles.restype = sz
@@ -430,6 +437,45 @@ class GrainsGPU:
         self._check(L.grains_gpu_get_pairs(self.h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n.value,
                                            ctypes.byref(n)))
         return buf[: 2 * n.value].reshape(-1, 2).copy()
+
+    def particle_class(self, cls):
+        """Class (RawData particle type) of the particles added next."""
+        self._check(lib().grains_gpu_set_particle_class(self.h, int(cls)))
+
+    def velocity_max_mean(self):
+        """(max |v|, mean |v|) over the particles (AllComponents::ComputeMaxMeanVelocity)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().grains_gpu_velocity_max_mean(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def coordination_numbers(self):
+        """Distinct contacting components per particle (id order) at the last
+        force evaluation made with pair recording on."""
+        out = np.zeros(self.n, dtype=np.int32)
+        self._check(lib().grains_gpu_coordination_numbers(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
+
+    def write_rawdata(self, root, binary=False, truncate=False, coord=None):
+        """One RawData frame at the current time (files <root>_*.dat)."""
+        cp = None
+        if coord is not None:
+            coord = np.ascontiguousarray(coord, dtype=np.int32)
+            if coord.shape != (self.n,):
+                raise ValueError("coord needs one value per particle")
+            cp = coord.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        self._check(lib().grains_gpu_write_rawdata(self.h, os.fsencode(root), int(bool(binary)), int(bool(truncate)),
+                                                   cp))
+
+    def set_outputs(self, save_times=(), save_root=None, rawdata_root=None, rawdata_binary=False):
+        """Output settings of run() (<TimeSave>, <RestartFile Name>, <RawData>)."""
+        t = _c64(np.asarray(save_times, dtype=np.float64).reshape(-1))
+        self._check(lib().grains_gpu_set_outputs(self.h, _d(t), len(t), os.fsencode(save_root) if save_root else None,
+                                                 os.fsencode(rawdata_root) if rawdata_root else None,
+                                                 int(bool(rawdata_binary))))
+
+    def run(self, verbose=False):
+        """Grains::Simulation to the end time with its outputs."""
+        self._check(lib().grains_gpu_run(self.h, int(bool(verbose))))
 
     def contact_counts(self):
         """(directed contact records, of which on the Coulomb cap) of the last
