@@ -267,6 +267,8 @@ def main():
     b_alg = (B_ALG[args.precision] + B_PAIR * cbar) * args.n
     achieved = b_alg / (force_ms / 1e3) / 1e9
     skin, builds = g.neighbor_stats()
+    ngraphs, replays, why = g.graph_stats()
+    log(f"bench: step graphs {ngraphs} ({replays} replays) {why}")
     log(f"bench: force sweep {force_ms * 1e3:.1f} us/launch, neighbour search {detect_ms * 1e3:.1f} us/launch, "
         f"binning {bin_ms * 1e3:.1f} us/step, contacts/particle {cbar:.4f}, Verlet skin {skin:.3g} m, "
         f"{builds} list builds in {args.warmup + args.steps + args.profile_steps} steps")

@@ -166,6 +166,13 @@ int grains_gpu_profile_steps(grains_gpu_system* sys, int nsteps, double* out);
 
 int grains_gpu_synchronize(grains_gpu_system* sys);
 
+/* Step batches replay their fused LeapFrog steps from captured HIP graphs
+ * (64 steps each, captured on first use per buffer-parity state).  Reports
+ * the graphs held (-1: capture refused on this stack, plain launches; why
+ * receives the reason) and the replays so far.  No counterpart in the
+ * reference (launch mechanics of this library). */
+int grains_gpu_graph_stats(grains_gpu_system* sys, int* ngraphs, long* replays, char* why, size_t cap);
+
 /* Forces and torques at the current state (Grains::computeParticlesForceAndTorque,
  * Grains.cpp:459-482); results via grains_gpu_get_state(F, T). */
 int grains_gpu_compute_forces(grains_gpu_system* sys);

@@ -452,7 +452,10 @@ hipError_t launch_stats_fold(const double* part, int n, double* tot, double time
 // waves overlap the dependent loads of the list; the force sweep then
 // visits only these pairs (same pairs, same order, same arithmetic).
 __global__ __launch_bounds__(256) void k_vfilter(Particles P, Sorted S, Grid G, VerletState VS, int n, int cx,
-                                                 NbrList L, NbrList CL) {
+                                                 NbrList L, NbrList CL, DevStatus* st) {
+  // the list is current (rebuilt if it was stale): clear the stale flag for
+  // this step's drift
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rebuild = 0;
   const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const d3 x = ld3(P.x[cx], s);
@@ -614,9 +617,9 @@ static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, 
 }
 
 hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, const VerletState& V, int n, int cx,
-                          const NbrList& L, const NbrList& CL, hipStream_t stream) {
+                          const NbrList& L, const NbrList& CL, DevStatus* st, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vfilter, dim3(nblk(n, 256)), dim3(256), 0, stream, P, S, G, V, n, cx, L, CL);
+  hipLaunchKernelGGL(k_vfilter, dim3(nblk(n, 256)), dim3(256), 0, stream, P, S, G, V, n, cx, L, CL, st);
   return hipGetLastError();
 }
 

@@ -20,7 +20,7 @@ hipError_t launch_key(const Particles& P, const Sorted& S, const Grid& G, const 
 hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n, int cx, hipStream_t stream,
                       const int* gate = nullptr, int stamp = 0);
 // Verlet list build (keys on its grid, binning, candidates); stamp > 0: only
-// if the list was found stale by the launch with that stamp
+// if the list was found stale (DevStatus::rebuild == stamp)
 hipError_t launch_verlet_build(const Particles& P, const Sorted& S, const VerletState& V, int n, int cx,
                                DevStatus* st, const NbrList& L, int stamp, hipStream_t stream);
 int scan_blocks(int ncells);
@@ -28,7 +28,7 @@ int scan_blocks(int ncells);
 hipError_t launch_clear(int* p, int n, hipStream_t stream);
 // contacts of this step from the Verlet list L into CL
 hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, const VerletState& V, int n, int cx,
-                          const NbrList& L, const NbrList& CL, hipStream_t stream);
+                          const NbrList& L, const NbrList& CL, DevStatus* st, hipStream_t stream);
 // folds the per-block contact statistics of the last force launch over n
 // particles into the running totals (6 doubles), at the given time
 hipError_t launch_stats_fold(const double* part, int n, double* tot, double time, hipStream_t stream);

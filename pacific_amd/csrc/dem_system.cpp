@@ -46,6 +46,7 @@ double hooke_beta(double en) { return std::log(en) / std::sqrt(kPi * kPi + std::
 // Device allocation
 
 void grains_gpu_system::free_device() {
+  drop_graphs();
   for (void* p : allocations) (void)hipFree(p);
   allocations.clear();
   if (ev0) (void)hipEventDestroy(ev0);
@@ -343,12 +344,12 @@ int grains_gpu_system::mark(std::vector<hipEvent_t>& evs, size_t& n) {
 }
 
 // Binning after a drift; with a Verlet list, its rebuild, which runs only
-// if the drift (the launch with the current stamp) found the list stale.
+// if the drift found the list stale.
 int grains_gpu_system::bin_launch() {
   int e;
   if (prof && (e = mark(prof->bin_ev, prof->nbin))) return e;
   if (verlet)
-    HIPCHECK(this, launch_verlet_build(P, S, V, nparticles, cur & 1, d_status, nbr_list(), stamp, stream));
+    HIPCHECK(this, launch_verlet_build(P, S, V, nparticles, cur & 1, d_status, nbr_list(), kStale, stream));
   else
     HIPCHECK(this, launch_bin(P, S, grid, nparticles, cur & 1, stream));
   if (prof && (e = mark(prof->bin_ev, prof->nbin))) return e;
@@ -378,7 +379,7 @@ int grains_gpu_system::forces(int mode, int first_step, bool commit) {
   if (CL.count) {
     // this step's contacts from the Verlet list
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
-    HIPCHECK(this, launch_vfilter(P, S, grid, V, nparticles, cur & 1, L, CL, stream));
+    HIPCHECK(this, launch_vfilter(P, S, grid, V, nparticles, cur & 1, L, CL, d_status, stream));
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
   }
   if (prof && (e = mark(prof->force_ev, prof->nforce))) return e;
@@ -424,7 +425,11 @@ int grains_gpu_system::enqueue_steps_impl(int nsteps) {
     HIPCHECK(this, launch_move(P, S, grid, sp, d_status, 0.5 * cfg.dt, cfg.dt, vstate(next_stamp()), stream));
     if ((e = bin_launch())) return e;
     if (recording && (e = record_state())) return e;
-    for (int i = 1; i < nsteps; ++i) {
+    int i = 1;
+    if (graph_steps_allowed())
+      for (bool done = true; done && nsteps - i >= kGraphSteps; i += done ? kGraphSteps : 0)
+        if ((e = graph_chunk(done))) return e;
+    for (; i < nsteps; ++i) {
       force_time += cfg.dt;
       if ((e = forces(kModeFused, 0))) return e;
       if ((e = bin_launch())) return e;
@@ -444,6 +449,75 @@ int grains_gpu_system::enqueue_steps_impl(int nsteps) {
   }
   steps_done += nsteps;
   for (int i = 0; i < nsteps; ++i) time += cfg.dt;
+  return GRAINS_OK;
+}
+
+// Graph replay serves the plain time loop: no per-step host work (profile
+// events, contact records, statistics with their time stamps, adjoint
+// recording) may be part of it.
+bool grains_gpu_system::graph_steps_allowed() const {
+  return !prof && !recording && !record_pairs && !stats_on && cfg.scheme == GRAINS_LEAPFROG;
+}
+
+void grains_gpu_system::drop_graphs() {
+  for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  graphs.clear();
+}
+
+// kGraphSteps fused LeapFrog steps (forces, kicks, drift, binning or the
+// gated Verlet rebuild): captured from the launches of the plain loop the
+// first time a parity state is met, replayed afterwards.  The kernels read
+// every per-step quantity from device memory, so the captured arguments
+// hold for every replay that starts from the same (cur, mem.cur).
+// done = false (and no error): the steps could not be captured on this
+// stack; nothing was launched, the host state is as before, and the plain
+// loop carries on.
+int grains_gpu_system::graph_chunk(bool& done) {
+  done = false;
+  if (graphs_failed) return GRAINS_OK;
+  if (cfg.dt != graph_dt) {
+    drop_graphs();
+    graph_dt = cfg.dt;
+  }
+  const int key = (cur & 3) | ((mem.cur & 1) << 2);
+  for (auto& g : graphs)
+    if (g.first == key) {
+      for (int j = 0; j < kGraphSteps; ++j) force_time += cfg.dt;
+      HIPCHECK(this, hipGraphLaunch(g.second, stream));
+      ++graph_replays;
+      done = true;
+      return GRAINS_OK;
+    }
+  const int cur0 = cur, mem0 = mem.cur;
+  const double ft0 = force_time;
+  HIPCHECK(this, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  int e = GRAINS_OK;
+  for (int j = 0; j < kGraphSteps && !e; ++j) {
+    force_time += cfg.dt;
+    e = forces(kModeFused, 0);
+    if (!e) e = bin_launch();
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(stream, &graph);
+  hipGraphExec_t exec = nullptr;
+  hipError_t ie = hipErrorUnknown;
+  if (!e && ce == hipSuccess && graph && cur == cur0 && mem.cur == mem0)
+    ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) {
+    graph_error = e ? last_error : std::string(hipGetErrorString(ce != hipSuccess ? ce : ie));
+    // captured launches never ran: back to the state before the capture
+    cur = cur0;
+    mem.cur = mem0;
+    force_time = ft0;
+    graphs_failed = true;
+    (void)hipGetLastError();
+    return GRAINS_OK;
+  }
+  graphs.emplace_back(key, exec);
+  HIPCHECK(this, hipGraphLaunch(exec, stream));
+  ++graph_replays;
+  done = true;
   return GRAINS_OK;
 }
 
@@ -879,6 +953,16 @@ int grains_gpu_profile_steps(grains_gpu_system* s, int nsteps, double* out) {
   for (auto& ev : pr.detect_ev) (void)hipEventDestroy(ev);
   if (e) return e;
   return s->check_status();
+}
+
+int grains_gpu_graph_stats(grains_gpu_system* s, int* ngraphs, long* replays, char* why, size_t cap) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  if (ngraphs) *ngraphs = s->graphs_failed ? -1 : (int)s->graphs.size();
+  if (replays) *replays = s->graph_replays;
+  if (why && cap) {
+    std::snprintf(why, cap, "%s", s->graph_error.c_str());
+  }
+  return GRAINS_OK;
 }
 
 int grains_gpu_synchronize(grains_gpu_system* s) {

@@ -95,7 +95,7 @@ struct grains_gpu_system {
   dem::NbrList contact_list() const {
     return verlet ? dem::NbrList{d_ct_count, d_ct, kContactCap, nparticles} : dem::NbrList{};
   }
-  int stamp = 0;
+
   // contact statistics (AppCollision): per-block partials of the last force
   // launch and running totals; force_time = time of the next force launch
   // HookeMemory contact memory (allocated when a model has memory)
@@ -106,7 +106,12 @@ struct grains_gpu_system {
   double* d_stat_part = nullptr;
   double* d_stat_tot = nullptr;
   double force_time = 0.;
-  int next_stamp() { return stamp = stamp >= (1 << 30) ? 1 : stamp + 1; }
+  // stale flag of the Verlet list: set by the drift that finds the list
+  // stale, read by the rebuild launches, cleared by the next contact filter
+  // (the same kernel arguments every step, so that steps can be replayed
+  // from a captured graph)
+  static constexpr int kStale = 1;
+  int next_stamp() const { return kStale; }
   dem::VerletState vstate(int st) const {
     dem::VerletState v = V;
     v.stamp = verlet ? st : 0;
@@ -144,6 +149,18 @@ struct grains_gpu_system {
   void free_adjoint();
   int enqueue_steps(int nsteps);
   int enqueue_steps_impl(int nsteps);
+  // LeapFrog steps replayed from captured graphs, kGraphSteps fused steps
+  // per graph (even, so that the buffer parities return), one graph per
+  // parity state at capture (cur, mem.cur), all dropped when dt changes
+  static constexpr int kGraphSteps = 64;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  double graph_dt = 0.;
+  bool graphs_failed = false;  // capture refused on this stack: plain launches
+  std::string graph_error;     // why
+  long graph_replays = 0;
+  bool graph_steps_allowed() const;
+  int graph_chunk(bool& done);
+  void drop_graphs();
   int recover();
   int check_status();
 };

@@ -146,7 +146,7 @@ struct VerletState {
   double* xb[3];  // positions at the last build (storage order)
   double skin;
   double half2;   // (skin / 2)^2
-  int stamp;      // step stamp of this launch (> 0); 0 = no Verlet list
+  int stamp;      // > 0: the staleness check sets DevStatus::rebuild = stamp; 0 = no Verlet list; < 0 wrap only
   int pad;
 };
 
@@ -160,7 +160,7 @@ struct DevStatus {
   int n_recovered;   // ContactError recoveries
   int pad;
   double overlap_max;
-  int rebuild;       // step stamp of the last launch that found the Verlet list stale
+  int rebuild;       // Verlet list found stale (the drift's stamp), cleared by the contact filter
   int n_rebuild;     // Verlet list builds
 };
 

@@ -49,6 +49,7 @@ EXPORTS = [
     "grains_gpu_neighbor_stats", "grains_gpu_set_contact_stats", "grains_gpu_contact_stats",
     "grains_gpu_set_memory_contact_model", "grains_gpu_set_particle_class", "grains_gpu_velocity_max_mean",
     "grains_gpu_coordination_numbers", "grains_gpu_write_rawdata", "grains_gpu_set_outputs", "grains_gpu_run",
+    "grains_gpu_graph_stats",
 ]
 
 
@@ -90,6 +91,7 @@ def lib():
         L.grains_gpu_write_rawdata.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ip]
         L.grains_gpu_set_outputs.argtypes = [vp, dp, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.grains_gpu_run.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_graph_stats.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_long), ctypes.c_char_p, sz]
         L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
         L.grains_gpu_num_particles.argtypes = [vp]
         L.grains_gpu_num_particles.restype = sz
@@ -376,6 +378,13 @@ class GrainsGPU:
 
     def synchronize(self):
         self._check(lib().grains_gpu_synchronize(self.h))
+
+    def graph_stats(self):
+        """(graphs held or -1 if capture is refused, replays, reason)"""
+        n, r = ctypes.c_int(), ctypes.c_long()
+        why = ctypes.create_string_buffer(256)
+        self._check(lib().grains_gpu_graph_stats(self.h, ctypes.byref(n), ctypes.byref(r), why, 256))
+        return n.value, r.value, why.value.decode()
 
     def compute_forces(self):
         self._check(lib().grains_gpu_compute_forces(self.h))
