@@ -75,6 +75,46 @@ def cpu_baseline(wl, steps):
     return wl["radius"].size * steps / (t1 - t0), t1 - t0
 
 
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0))}
+
+
+def _oracle_member(job):
+    """One ensemble system on the CPU oracle (a worker process)."""
+    from oracle.oracle import Oracle
+    from pacific_amd import workloads
+    wl, steps = job
+    o = Oracle(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], 0.0, wl["dt"], 0)
+    workloads.build(o, wl)
+    o.init()
+    o.step(steps)
+    return wl["radius"].size * steps
+
+
+def ensemble_cpu_baseline(nsys, steps, cores):
+    """The CPU oracle over the first nsys systems of the batched ensemble,
+    one system per worker process on `cores` cores (the reference runs one
+    system per process).  Runs before anything touches the GPU: the workers
+    are forked."""
+    import multiprocessing as mp
+    from pacific_amd import workloads
+    ens = workloads.ensemble(512, seed=5)[:nsys]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        work = sum(pool.map(_oracle_member, [(wl, steps) for wl in ens], chunksize=1))
+    dt = time.perf_counter() - t0
+    return work / dt, dt
+
+
 def batched_leg(args, prec):
     """512 independent settling boxes (sizes log-uniform in [200, 5000],
     per-system mu and en) in one persistent launch per hipGraph replay, the
@@ -197,6 +237,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-ensemble-systems", type=int, default=64,
+                    help="ensemble systems in the all-cores CPU leg (0: skip it)")
+    ap.add_argument("--cpu-ensemble-steps", type=int, default=1000)
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--nbr-cap", type=int, default=32,
                     help="neighbour candidates per particle of the per-step list (0: 27-cell sweep in the force kernel)")
@@ -244,6 +287,18 @@ def main():
         from pacific_amd import FP64, MIXED
         print(json.dumps(polyhedra_leg(args, MIXED if args.precision == "mixed" else FP64)), flush=True)
         return
+
+    # the all-cores CPU leg of the ensemble first, while no process has
+    # touched the GPU (its workers are forked)
+    ens_cpu = None
+    if not args.no_cpu and not args.no_batch and args.cpu_ensemble_systems > 0:
+        cores = min(16, len(os.sched_getaffinity(0)))
+        v, t = ensemble_cpu_baseline(args.cpu_ensemble_systems, args.cpu_ensemble_steps, cores)
+        ens_cpu = {"value": v, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                   "sample": f"the first {args.cpu_ensemble_systems} of the 512 ensemble systems, "
+                             f"{args.cpu_ensemble_steps} steps each, one system per worker process "
+                             f"({t:.1f} s wall), FP64 oracle"}
+        log(f"bench: CPU oracle on the ensemble {v:.4g} particle-steps/s on {cores} cores")
 
     from pacific_amd import FP64, MIXED, workloads
     wl = workloads.periodic_gas(args.n, seed=1)
@@ -318,8 +373,11 @@ def main():
         cpu_value, cpu_s = cpu_baseline(wl, args.cpu_steps)
         out["cpu_baseline"] = {"value": cpu_value, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                                "sample": f"the same {args.n}-particle gas, {args.cpu_steps} steps "
-                                         f"({cpu_s:.1f} s), FP64 serial oracle restating the reference step"}
+                                         f"({cpu_s:.1f} s), FP64 serial oracle restating the reference step",
+                               "host": host_cpu()}
         log(f"bench: CPU oracle {cpu_value:.4g} particle-steps/s on 1 core")
+        if ens_cpu is not None:
+            out["batched_cpu_baseline"] = ens_cpu
     print(json.dumps(out), flush=True)
 
 

@@ -36,8 +36,11 @@ Workloads with non-spherical particles carry ``shapes`` (kind, radius, L,
 verts, faces, crust) and ``groups`` (shape index, x, v, w, basis, density,
 material) in insertion order.
 """
-from pacific_amd import shapes as _shapes
+import os
+
 import numpy as np
+
+from pacific_amd import shapes as _shapes
 
 
 def settling_box(n=1000, seed=12345, radius=1e-3, density=2500.0, crust=5e-5, kn=1e4, en=0.9, muc=0.5, kr=0.0,
@@ -334,3 +337,97 @@ def _merged_groups(groups):
         else:
             out.append((shape, x, v, w, B, rho, mat))
     return out
+
+
+def write_input(wl, directory, t_end, name="insert", dt=None):
+    """Writes a single-class sphere workload as a Grains3D input: an XML file
+    with a <ParticlePosition><File> insertion and the positions file next to
+    it (the layout the reference's own tests use, e.g. spheres-settling-new-
+    grains/Grains/Init/insert.xml).  The particles start at rest; box
+    obstacles must be axis-aligned.  Returns the XML path."""
+    r, rho, cr = (np.unique(np.asarray(wl[k])) for k in ("radius", "density", "crust"))
+    if len(r) != 1 or len(rho) != 1 or len(cr) != 1 or len(np.unique(wl["material"])) != 1:
+        raise ValueError("write_input needs one particle class (equal radius, density, crust, material)")
+    if np.any(wl["v"]) or np.any(wl["w"]):
+        raise ValueError("write_input writes particles at rest")
+    if "shapes" in wl:
+        raise ValueError("write_input writes spheres only")
+    os.makedirs(directory, exist_ok=True)
+    pos = f"{name}_positions.result"
+    np.savetxt(os.path.join(directory, pos), wl["x"], fmt="%.17g")
+    mats = sorted({int(wl["material"][0])} | {int(b[4]) for b in wl["boxes"]})
+    mname = {m: f"mat{m}" for m in mats}
+    models = []
+    for m in wl["models"]:
+        if m[0] == "memory":
+            raise ValueError("write_input writes <Hooke> models only")
+        ma, mb, kn, en, muc, kr, etat = m
+        models.append(f"""      <ContactForceModel>
+        <Material materialA="{mname.get(ma, f'mat{ma}')}" materialB="{mname.get(mb, f'mat{mb}')}"/>
+        <Hooke><kn>{kn!r}</kn><en>{en!r}</en><muc>{muc!r}</muc><kr>{kr!r}</kr><etat>{etat!r}</etat></Hooke>
+      </ContactForceModel>""")
+    walls = []
+    for i, (L, c, B, crust, mat) in enumerate(wl["boxes"]):
+        if B is not None and not np.allclose(np.asarray(B, float).reshape(3, 3), np.eye(3)):
+            raise ValueError("write_input writes axis-aligned boxes only")
+        walls.append(f"""        <Obstacle name="wall{i}">
+          <Convex CrustThickness="{float(crust)!r}">
+            <Box LX="{float(L[0])!r}" LY="{float(L[1])!r}" LZ="{float(L[2])!r}"/>
+          </Convex>
+          <Centre X="{float(c[0])!r}" Y="{float(c[1])!r}" Z="{float(c[2])!r}"/>
+          <AngularPosition Type="Identity"></AngularPosition>
+          <Material>{mname[int(mat)]}</Material>
+        </Obstacle>""")
+    o, hi, per = wl["origin"], wl["max"], wl["periodic"]
+    g = wl["gravity"]
+    text = f"""<?xml version="1.0" encoding="iso-8859-1"?>
+<Grains3D Type="Standard">
+  <Construction>
+    <LinkedCell MX="{float(hi[0])!r}" MY="{float(hi[1])!r}" MZ="{float(hi[2])!r}" CellSizeFactor="{float(wl['cellsize_factor'])!r}"/>
+    <Origin OX="{float(o[0])!r}" OY="{float(o[1])!r}" OZ="{float(o[2])!r}"/>
+    <Periodicity PX="{int(per[0])}" PY="{int(per[1])}" PZ="{int(per[2])}"/>
+    <CollisionDetectionAlgorithm>
+      <CollisionDetection Type="GJK" Tolerance="1.e-10"/>
+    </CollisionDetectionAlgorithm>
+    <ContactForceModels>
+{chr(10).join(models)}
+    </ContactForceModels>
+    <Particles>
+      <Particle Number="{len(wl['x'])}" Density="{float(rho[0])!r}">
+        <Convex CrustThickness="{float(cr[0])!r}">
+          <Sphere Radius="{float(r[0])!r}"/>
+        </Convex>
+        <AngularPosition Type="Identity"></AngularPosition>
+        <Material>{mname[int(wl['material'][0])]}</Material>
+      </Particle>
+    </Particles>
+    <Obstacles>
+      <Composite name="Box">
+{chr(10).join(walls)}
+      </Composite>
+    </Obstacles>
+  </Construction>
+  <Forces>
+    <Gravity GX="{float(g[0])!r}" GY="{float(g[1])!r}" GZ="{float(g[2])!r}"/>
+  </Forces>
+  <Simulation>
+    <TimeInterval Start="0." End="{float(t_end)!r}"/>
+    <TimeStep dt="{float(dt if dt is not None else wl['dt'])!r}"/>
+    <TimeIntegration Type="SecondOrderLeapFrog"/>
+    <ParticleInsertion>
+      <Mode Type="InitialTime"/>
+      <Order Type="Ordered"/>
+      <InitialAngularPosition Type="Fixed as defined in input file particle class"/>
+      <RandomGeneratorSeed Type="Default"/>
+      <ParticlePosition>
+        <File Name="{pos}"/>
+      </ParticlePosition>
+      <InitialVelocity Mode="Zero"/>
+    </ParticleInsertion>
+  </Simulation>
+</Grains3D>
+"""
+    path = os.path.join(directory, f"{name}.xml")
+    with open(path, "w") as f:
+        f.write(text)
+    return path

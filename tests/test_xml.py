@@ -146,3 +146,25 @@ def test_xml_equals_programmatic_build(tmp_path, gpu):
     s1, s2 = g1.state(), g2.state()
     for k in ("x", "v", "w", "q"):
         assert np.array_equal(s1[k], s2[k]), k
+
+
+def test_written_input_parses(tmp_path):
+    # the settling box (C1 shape) written as a Grains3D input by the package
+    path = workloads.write_input(workloads.settling_box(300), str(tmp_path / "c1"), t_end=0.01)
+    assert os.path.exists(os.path.join(os.path.dirname(path), "insert_positions.result"))
+    if gpu_available():
+        assert GrainsGPU.from_xml(path).n == 300
+    else:
+        expect_error(path, "DeviceError", "")
+
+
+@pytest.mark.gpu
+def test_written_input_equals_programmatic_build(tmp_path, gpu):
+    wl = workloads.settling_box(300)
+    g1 = GrainsGPU.from_xml(workloads.write_input(wl, str(tmp_path / "c1"), t_end=0.01))
+    g2 = make_gpu(wl, record=False)
+    g1.step(200)
+    g2.step(200)
+    s1, s2 = g1.state(), g2.state()
+    for k in ("x", "v", "w", "q"):
+        assert np.array_equal(s1[k], s2[k]), k
