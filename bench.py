@@ -237,9 +237,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--cpu-ensemble-systems", type=int, default=64,
+    ap.add_argument("--cpu-ensemble-systems", type=int, default=128,
                     help="ensemble systems in the all-cores CPU leg (0: skip it)")
-    ap.add_argument("--cpu-ensemble-steps", type=int, default=1000)
+    ap.add_argument("--cpu-ensemble-steps", type=int, default=5000)
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--nbr-cap", type=int, default=32,
                     help="neighbour candidates per particle of the per-step list (0: 27-cell sweep in the force kernel)")

@@ -425,6 +425,30 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
       for (int a = 0; a < 3; ++a) cfg.periodic[a] = std::atoi(p->attr(ax[a]).c_str()) == 1 ? 1 : 0;
     }
     if (cons->child("PartialPeriodicity")) c.bad("<PartialPeriodicity> is not supported");
+    // <CollisionDetectionAlgorithm> (Grains.cpp:640-770): Johnson's GJK at
+    // the default relative tolerance 1e-10, without acceleration or history,
+    // is what this library runs; other choices are refused rather than
+    // ignored.  A <BoundingVolume> (OBB / OBC) is only a pre-test that skips
+    // GJK for separated pairs, so the contacts are the same without it.
+    if (const XNode* cda = cons->child("CollisionDetectionAlgorithm"))
+      if (const XNode* cd = cda->child("CollisionDetection")) {
+        if (cd->has("Type")) {
+          const std::string t = cd->attr("Type");
+          if (t == "GJK_SV") c.bad("<CollisionDetection Type=\"GJK_SV\"> is not supported (GJK only)");
+          else if (t != "GJK") c.bad("collision detection algorithm " + t + " is not defined");
+        }
+        if (c.ok && cd->has("Tolerance")) {
+          const double tol = dattr(cd, "Tolerance", c);
+          if (c.ok && tol < 1e-10) c.bad("GJK tolerance should be at least 1E-10");
+          else if (c.ok && tol != 1e-10) c.bad("only the default GJK tolerance 1e-10 is supported");
+        }
+        for (const char* opt_name : {"Acceleration", "History"}) {
+          if (!c.ok || !cd->has(opt_name)) continue;
+          const std::string v = cd->attr(opt_name);
+          if (v == "ON") c.bad(std::string("GJK ") + opt_name + "=\"ON\" is not supported");
+          else if (v != "OFF") c.bad(std::string(opt_name) + " should be ON or OFF");
+        }
+      }
     if (cons->child("Reload")) c.bad("<Reload> is not supported by the XML front-end; use grains_gpu_set_state");
     if (cons->child("CompositeParticles")) c.bad("<CompositeParticles> are not supported");
     if (const XNode* ps = cons->child("Particles")) {

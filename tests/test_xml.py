@@ -120,6 +120,11 @@ def test_valid_file_parses(tmp_path):
     (("<TimeStep dt=\"1.e-5\"/>", "<TimeStep/>"), "missing attribute dt"),
     (("</Construction>", "</Construktion>"), "does not match"),
     (("positions.result", "missing.result"), "cannot open position file"),
+    (('Type="GJK"', 'Type="GJK_SV"'), "GJK_SV"),
+    (('Tolerance="1.e-10"', 'Tolerance="1.e-6"'), "default GJK tolerance"),
+    (('Tolerance="1.e-10"', 'Tolerance="1.e-12"'), "at least 1E-10"),
+    (('Tolerance="1.e-10"', 'Tolerance="1.e-10" Acceleration="ON"'), "Acceleration"),
+    (('Tolerance="1.e-10"', 'Tolerance="1.e-10" History="maybe"'), "ON or OFF"),
 ])
 def test_rejections(tmp_path, sub, frag):
     path, _ = write_case(tmp_path, **dict(zip(sub[::2], sub[1::2])))
