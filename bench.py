@@ -191,7 +191,7 @@ def polyhedra_leg(args, prec):
     """200,000 polyhedra (thirds of cubes, tetrahedra, icosahedra) in a
     silo, random orientations and velocities: GJK narrow phase on
     crust-shrunk shapes, ParticleKinematics3D, basis composition."""
-    from pacific_amd import workloads
+    from pacific_amd import FP64, workloads
     wl = workloads.silo(args.poly_n)
     g = build_gpu(wl, prec)
     g.step(args.poly_warmup)
@@ -204,6 +204,12 @@ def polyhedra_leg(args, prec):
     g.record_pairs(False)
     prof = g.profile_steps(min(args.poly_steps, 100))
     force_ms = prof["force_ms"] / max(prof["n_force"], 1)
+    ferr = None
+    if prec != FP64:
+        g64 = build_gpu(wl, FP64)
+        ferr = force_error(g, g64, shapes=True)
+        del g64
+        log(f"bench: polyhedra mixed vs FP64 force error {ferr['rel_to_max_F']:.2e} (relative to the largest force)")
     log(f"bench: polyhedra silo {args.poly_n}: {value:.4g} particle-steps/s, force sweep {force_ms * 1e3:.1f} us, "
         f"contacts/particle {cbar:.3f}")
     return {"polyhedra_value": value,
@@ -211,7 +217,44 @@ def polyhedra_leg(args, prec):
                                              "icosahedron R 1.2 mm), random orientations, sigma_v 0.5 m/s, "
                                              "GJK, dt 5e-6", "steps": args.poly_steps,
                                  "ms_per_step": ms / args.poly_steps, "force_sweep_ms": force_ms,
-                                 "contacts_per_particle": cbar}}
+                                 "contacts_per_particle": cbar, "mixed_force_error": ferr}}
+
+
+def force_error(gm, g64, shapes=False):
+    """Mixed-precision force error at one state: the FP64 system is put in
+    the mixed system's state and both evaluate forces there.  Returns the
+    largest force and torque deviations relative to the largest force /
+    torque of the FP64 evaluation, and the largest per-particle deviation
+    relative to that particle's contact-force scale (|F| with a floor of
+    1e-6 of the largest)."""
+    s = gm.state()
+    if shapes:
+        g64.set_basis(gm.basis())
+    g64.set_state(x=s["x"], v=s["v"], w=s["w"], q=s["q"], recompute=True)
+    gm.compute_forces()
+    a, b = gm.state(), g64.state()
+    out = {}
+    for k in ("F", "T"):
+        d = np.linalg.norm(a[k] - b[k], axis=1)
+        m = np.linalg.norm(b[k], axis=1)
+        top = float(m.max()) if m.size else 0.0
+        out[f"max_abs_{k}"] = float(d.max()) if d.size else 0.0
+        out[f"rel_to_max_{k}"] = float(d.max() / top) if top > 0 else 0.0
+        out[f"max_rel_per_particle_{k}"] = float(np.max(d / np.maximum(m, 1e-6 * top))) if top > 0 else 0.0
+    return out
+
+
+def settling_force_error():
+    """C1 (1,000 spheres settling in a box of walls) run to rest in FP64,
+    then the mixed force error at that state."""
+    from pacific_amd import FP64, MIXED, workloads
+    wl = workloads.settling_box(1000)
+    g64 = build_gpu(wl, FP64)
+    g64.step(15000)
+    gm = build_gpu(wl, MIXED)
+    s = g64.state()
+    gm.set_state(x=s["x"], v=s["v"], w=s["w"], q=s["q"], recompute=True)
+    return force_error(gm, g64)
 
 
 def read_traffic(name):
@@ -335,7 +378,12 @@ def main():
         ms64 = g64.step_timed(max(args.steps // 2, 1))
         extra["fp64_value"] = args.n * max(args.steps // 2, 1) / (ms64 / 1e3)
         extra["fp64_ms_per_step"] = ms64 / max(args.steps // 2, 1)
+        fe = {"gas": force_error(g, g64)}
         del g64
+        fe["settling_box_1000_at_rest"] = settling_force_error()
+        extra["mixed_force_error"] = fe
+        log(f"bench: mixed vs FP64 force error: gas {fe['gas']['rel_to_max_F']:.2e}, settled box "
+            f"{fe['settling_box_1000_at_rest']['rel_to_max_F']:.2e} (relative to the largest force)")
         log(f"bench: FP64 path {extra['fp64_value']:.4g} particle-steps/s")
 
     if not args.no_batch:
