@@ -194,7 +194,7 @@ def adjoint_leg(args):
                                "dJ_dv0_norm": float(np.linalg.norm(grad["v0"])),
                                "kinetic_energy": {"value": ke, "dJ_dparams": [float(x) for x in gke["params"]],
                                                   "dJ_dv0_norm": float(np.linalg.norm(gke["v0"]))},
-                               "params": "kn, en, muc of the sphere-sphere pair"}}
+                               "params": "kn, en, muc of material pair (0, 0): sphere-sphere and sphere-plane"}}
 
 
 def polyhedra_leg(args, prec):
