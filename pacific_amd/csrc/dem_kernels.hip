@@ -152,12 +152,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(int* __restrict__ blo
   }
 }
 
+// FOLD: block_sums holds the tiles' totals (no k_scan_top launch) and each
+// block adds those before it itself -- for grids of at most kScanFoldTiles
+// tiles, where the sums are cheaper than a launch.
+constexpr int kScanFoldTiles = 4096;
+
+template <bool FOLD>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int* __restrict__ count, int ncells,
                                                             const int* __restrict__ block_sums,
                                                             int* __restrict__ start, int n, const int* gate,
                                                             int stamp) {
   if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
   __shared__ int lds[kScanThreads];
+  int offset;
+  if (FOLD) {
+    int part = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) part += block_sums[b];
+    block_exclusive_scan(part, lds, offset);
+  } else {
+    offset = block_sums[blockIdx.x];
+  }
   int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int vals[kScanItems];
   int sum = 0;
@@ -167,7 +181,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int* __restric
     sum += vals[i];
   }
   int total;
-  int run = block_exclusive_scan(sum, lds, total) + block_sums[blockIdx.x];
+  int run = block_exclusive_scan(sum, lds, total) + offset;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     if (base + i < ncells) start[base + i] = run;
@@ -522,9 +536,14 @@ hipError_t launch_bin(const Particles& P, const Sorted& S, const Grid& G, int n,
   int nb = nblk(G.ncells, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums, gate,
                      stamp);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, S.block_sums, nb, gate, stamp);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums,
-                     S.start, n, gate, stamp);
+  if (nb <= kScanFoldTiles) {
+    hipLaunchKernelGGL(k_scan_down<true>, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums,
+                       S.start, n, gate, stamp);
+  } else {
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, S.block_sums, nb, gate, stamp);
+    hipLaunchKernelGGL(k_scan_down<false>, dim3(nb), dim3(kScanThreads), 0, stream, S.count, G.ncells, S.block_sums,
+                       S.start, n, gate, stamp);
+  }
   // gated launches mostly find nothing to do: small grids that stride
   const int cap = gate ? kGatedBlocks : (1 << 30);
   if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(std::min(nblk(n, 256), cap)), dim3(256), 0, stream, S, n, gate, stamp);

@@ -34,6 +34,10 @@ std::string sci(double x, int digits) {
   return buf;
 }
 
+// Steps per call of the run loop between outputs (bounds the int argument
+// of grains_gpu_step on long runs without output times).
+constexpr long kRunChunk = 1L << 20;
+
 // Field files of the RawData writer, in the order the reference opens them
 // (RawDataPostProcessingWriter.cpp:327-358).
 const char* const kFields[10] = {"_position_x.dat",
@@ -233,7 +237,8 @@ int grains_gpu_run(grains_gpu_system* s, int verbose) {
     }
     // the contact records of the step that ends at an output time give the
     // coordination numbers; the other steps run without records
-    if (k > 1 && (e = grains_gpu_set_record_pairs(s, rec0)) == GRAINS_OK) e = grains_gpu_step(s, (int)(k - 1));
+    e = grains_gpu_set_record_pairs(s, rec0);
+    for (long left = k - 1; !e && left > 0; left -= kRunChunk) e = grains_gpu_step(s, (int)std::min(left, kRunChunk));
     if (!e && out && raw) e = grains_gpu_set_record_pairs(s, 1);
     if (!e) e = grains_gpu_step(s, 1);
     if (!e && out && raw) e = grains_gpu_coordination_numbers(s, coord.data());
