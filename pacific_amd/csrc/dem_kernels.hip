@@ -423,6 +423,16 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   if (sp.stats) block_stats(acc, sp.stats + 5 * (size_t)blockIdx.x);
 }
 
+// FP64 spheres without extras: register use capped so that three waves per
+// SIMD fit (the uncapped allocation leaves room for two).
+template <int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_forces64(
+    Particles P, Sorted S, Grid G, const BoxObstacle* __restrict__ obs, const HookeModel* __restrict__ models,
+    StepParams sp, DevStatus* st, int* pairs, NbrList L, VerletState VS, NbrList CL) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t < sp.n) force_one<MODE, false, false, false>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, nullptr);
+}
+
 // Running totals of the contact statistics after a force launch of nb
 // blocks: tot = {overlap sum, overlap max, time of the max, contacts, GJK
 // iterations, evaluations with iterations}; the maximum keeps the first
@@ -629,6 +639,8 @@ static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, 
     if (sp.stats || sp.mem.cap)
       hipLaunchKernelGGL((k_forces<MODE, MIXED, false, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
                          pairs, L, V, CL);
+    else if (!MIXED)
+      hipLaunchKernelGGL((k_forces64<MODE>), grid, block, 0, stream, P, S, G, obs, models, sp, st, pairs, L, V, CL);
     else
       hipLaunchKernelGGL((k_forces<MODE, MIXED, false, false>), grid, block, 0, stream, P, S, G, obs, models, sp,
                          st, pairs, L, V, CL);
