@@ -422,7 +422,7 @@ int grains_gpu_adjoint_backward(grains_gpu_system* s, int matA, int matB, const 
       Sr.order = di;
       Sr.sid = di + n;
       Sr.start = di + 2 * n;
-      hipLaunchKernelGGL(k_adj_sweep, dim3(nblk(N, 256)), dim3(256), 0, s->stream, Pr, Sr, s->grid, s->d_obs,
+      hipLaunchKernelGGL(k_adj_sweep, dim3(nblk(N, 64)), dim3(64), 0, s->stream, Pr, Sr, s->grid, s->d_obs,
                          (int)s->obstacles.size(), s->d_models, A, N, 0, matA, matB, h, dt, step >= 1 ? 2. : 1., 0,
                          s->d_status);
       ACHECK(s, hipGetLastError());
@@ -434,7 +434,7 @@ int grains_gpu_adjoint_backward(grains_gpu_system* s, int matA, int matB, const 
   }
   // the initial force evaluation f(x_0, v_0, w_0)
   if ((e = load_state(s, ad->ckpt[0]))) return e;
-  hipLaunchKernelGGL(k_adj_sweep, dim3(nblk(N, 256)), dim3(256), 0, s->stream, s->P, s->S, s->grid, s->d_obs,
+  hipLaunchKernelGGL(k_adj_sweep, dim3(nblk(N, 64)), dim3(64), 0, s->stream, s->P, s->S, s->grid, s->d_obs,
                      (int)s->obstacles.size(), s->d_models, A, N, 0, matA, matB, h, dt, 1., 1, s->d_status);
   ACHECK(s, hipGetLastError());
   ACHECK(s, hipStreamSynchronize(s->stream));

@@ -633,11 +633,12 @@ template <int MODE, bool MIXED>
 static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
                             const HookeModel* models, const StepParams& sp, DevStatus* st, int* pairs,
                             const NbrList& L, const VerletState& V, const NbrList& CL, hipStream_t stream) {
-  const dim3 grid(nblk(sp.n, 256)), block(256);
-  // the plain sphere sweeps run one wave per workgroup: at 2-3 waves per
-  // SIMD, whole-wave workgroups fill the CUs more evenly than 4-wave ones
-  // (92.7 -> 85.7 us on the 1M gas); the statistics kernels keep 256
-  const dim3 grid1(nblk(sp.n, kSphereBlock)), block1(kSphereBlock);
+  // one wave per workgroup: at 1-3 waves per SIMD, whole-wave workgroups
+  // fill the CUs more evenly than 4-wave ones (the mixed sphere sweep of the
+  // 1M gas: 92.7 us with 256 threads, 85.7 with 64); a launch that collects
+  // statistics keeps 256 threads, the granularity of its partial sums
+  const int bs = sp.stats ? 256 : kSphereBlock;
+  const dim3 grid(nblk(sp.n, bs)), block(bs);
   if (P.shape) {
     hipLaunchKernelGGL((k_forces<MODE, MIXED, true, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
                        pairs, L, V, CL);
@@ -646,9 +647,9 @@ static void launch_forces_t(const Particles& P, const Sorted& S, const Grid& G, 
       hipLaunchKernelGGL((k_forces<MODE, MIXED, false, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
                          pairs, L, V, CL);
     else if (!MIXED)
-      hipLaunchKernelGGL((k_forces64<MODE>), grid1, block1, 0, stream, P, S, G, obs, models, sp, st, pairs, L, V, CL);
+      hipLaunchKernelGGL((k_forces64<MODE>), grid, block, 0, stream, P, S, G, obs, models, sp, st, pairs, L, V, CL);
     else
-      hipLaunchKernelGGL((k_forces<MODE, MIXED, false, false>), grid1, block1, 0, stream, P, S, G, obs, models, sp,
+      hipLaunchKernelGGL((k_forces<MODE, MIXED, false, false>), grid, block, 0, stream, P, S, G, obs, models, sp,
                          st, pairs, L, V, CL);
   }
 }
