@@ -588,8 +588,9 @@ __global__ __launch_bounds__(256) void k_vkey(Particles P, Sorted S, VerletState
 __global__ __launch_bounds__(256) void k_vdetect(Sorted S, VerletState V, int n, NbrList L, const int* gate,
                                                  int stamp) {
   if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
-  for (int blk = xcd_tile(blockIdx.x, gridDim.x); blk * 256 < n; blk += gridDim.x) {
-    const int k = blk * 256 + threadIdx.x;
+  const int bs = blockDim.x;
+  for (int blk = xcd_tile(blockIdx.x, gridDim.x); blk * bs < n; blk += gridDim.x) {
+    const int k = blk * bs + threadIdx.x;
     if (k >= n) break;
     const int s = S.order[k];
     if ((unsigned)s >= (unsigned)n) continue;
@@ -621,7 +622,10 @@ hipError_t launch_verlet_build(const Particles& P, const Sorted& S, const Verlet
   const int* gate = stamp ? &st->rebuild : nullptr;
   hipError_t e = launch_bin(P, S, V.Gv, n, cx, stream, gate, stamp);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_vdetect, dim3(nb), dim3(256), 0, stream, S, V, n, L, gate, stamp);
+  // one wave per workgroup (the candidate loops differ in length from
+  // particle to particle)
+  const int nb1 = stamp ? std::min(nblk(n, 64), 4 * kGatedBlocks) : nblk(n, 64);
+  hipLaunchKernelGGL(k_vdetect, dim3(nb1), dim3(64), 0, stream, S, V, n, L, gate, stamp);
   return hipGetLastError();
 }
 
