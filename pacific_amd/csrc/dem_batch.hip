@@ -423,7 +423,7 @@ int grains_gpu_batch_create(grains_gpu_system* const* systems, int nsys, int pre
 void grains_gpu_batch_destroy(grains_gpu_batch* b) { delete b; }
 
 int grains_gpu_batch_set_block(grains_gpu_batch* b, int block) {
-  if (!b || (block != 64 && block != 128 && block != 256 && block != 512)) return GRAINS_ARGUMENT_ERROR;
+  if (!b || block < 64 || block > 512 || block % 64) return GRAINS_ARGUMENT_ERROR;
   if (block != b->block && b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
@@ -507,14 +507,17 @@ int grains_gpu_batch_step_timed(grains_gpu_batch* b, int nsteps, int chunk, doub
 }
 
 // Times `trial_steps` steps at every workgroup size and keeps the fastest;
-// ms_out[i] receives the time of candidate i (64, 128, 256, 512).
+// ms_out[i] receives the time of candidate i (64, 128, ..., 512: whole
+// waves, so that sizes between the powers of two can fill the wave slots a
+// CU has at the kernel's register count).
 int grains_gpu_batch_autotune(grains_gpu_batch* b, int trial_steps, int* best_block, double* ms_out) {
   if (!b || trial_steps <= 0) return GRAINS_ARGUMENT_ERROR;
   BCHECK(b, hipSetDevice(b->device));
-  const int cand[4] = {64, 128, 256, 512};
+  constexpr int kCand = 8;
   double best = 1e300;
   int bestb = b->block;
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < kCand; ++i) {
+    const int cand[kCand] = {64, 128, 192, 256, 320, 384, 448, 512};
     grains_gpu_batch_set_block(b, cand[i]);
     BCHECK(b, hipEventRecord(b->ev0, b->stream));
     int e = b->launch(1, trial_steps);

@@ -569,9 +569,9 @@ class GrainsBatch:
 
     def autotune(self, trial_steps=20):
         best = ctypes.c_int()
-        ms = np.zeros(4)
+        ms = np.zeros(8)
         self._check(lib().grains_gpu_batch_autotune(self.h, int(trial_steps), ctypes.byref(best), _d(ms)))
-        return best.value, dict(zip((64, 128, 256, 512), ms.tolist()))
+        return best.value, dict(zip((64, 128, 192, 256, 320, 384, 448, 512), ms.tolist()))
 
     def state(self, i):
         n = self.sizes[i]

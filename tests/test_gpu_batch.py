@@ -61,6 +61,6 @@ def test_member_against_oracle(gpu):
 def test_autotune_picks_a_candidate(gpu):
     batch = GrainsBatch([unconfigured(w) for w in members(8, seed=2)])
     best, ms = batch.autotune(10)
-    assert best in (64, 128, 256, 512)
+    assert best in (64, 128, 192, 256, 320, 384, 448, 512) and len(ms) == 8
     assert all(t > 0 for t in ms.values())
     assert ms[best] == min(ms.values())
