@@ -339,12 +339,14 @@ def _merged_groups(groups):
     return out
 
 
-def write_input(wl, directory, t_end, name="insert", dt=None):
+def write_input(wl, directory, t_end, name="insert", dt=None, save_every=None):
     """Writes a single-class sphere workload as a Grains3D input: an XML file
     with a <ParticlePosition><File> insertion and the positions file next to
     it (the layout the reference's own tests use, e.g. spheres-settling-new-
     grains/Grains/Init/insert.xml).  The particles start at rest; box
-    obstacles must be axis-aligned.  Returns the XML path."""
+    obstacles must be axis-aligned.  save_every: output times every that
+    many seconds (<TimeSave>), with <RestartFile> and <RawData> roots
+    directory/name.  Returns the XML path."""
     r, rho, cr = (np.unique(np.asarray(wl[k])) for k in ("radius", "density", "crust"))
     if len(r) != 1 or len(rho) != 1 or len(cr) != 1 or len(np.unique(wl["material"])) != 1:
         raise ValueError("write_input needs one particle class (equal radius, density, crust, material)")
@@ -378,6 +380,17 @@ def write_input(wl, directory, t_end, name="insert", dt=None):
           <AngularPosition Type="Identity"></AngularPosition>
           <Material>{mname[int(mat)]}</Material>
         </Obstacle>""")
+    outputs = ""
+    if save_every:
+        root = os.path.join(directory, name)
+        outputs = f"""
+    <RestartFile Name="{root}" WritingMode="Text"/>
+    <TimeSave Start="0." End="{float(t_end)!r}" Every="{float(save_every)!r}"/>
+    <PostProcessing>
+      <Writers>
+        <RawData Name="{root}"/>
+      </Writers>
+    </PostProcessing>"""
     o, hi, per = wl["origin"], wl["max"], wl["periodic"]
     g = wl["gravity"]
     text = f"""<?xml version="1.0" encoding="iso-8859-1"?>
@@ -413,7 +426,7 @@ def write_input(wl, directory, t_end, name="insert", dt=None):
   <Simulation>
     <TimeInterval Start="0." End="{float(t_end)!r}"/>
     <TimeStep dt="{float(dt if dt is not None else wl['dt'])!r}"/>
-    <TimeIntegration Type="SecondOrderLeapFrog"/>
+    <TimeIntegration Type="SecondOrderLeapFrog"/>{outputs}
     <ParticleInsertion>
       <Mode Type="InitialTime"/>
       <Order Type="Ordered"/>
