@@ -156,3 +156,20 @@ def test_run_with_outputs_equals_plain_steps(gpu, tmp_path):
     assert last == _expected_line(h.time, sh["x"][:, 2])
     assert np.array_equal(r["coordination"][-1], h.coordination_numbers())
     assert np.array_equal(r["coordination"][0], np.zeros(200))
+
+
+@pytest.mark.gpu
+def test_run_ends_with_a_monitor_line_off_the_output_times(gpu, tmp_path):
+    # outputs at 0.002 and 0.004, the run ends at 0.005: the end of the run
+    # adds its own monitor line (Grains.cpp:199-211) but no RawData frame
+    root = str(tmp_path / "o")
+    path, _ = write_case(tmp_path, n=100, **{'End="0.1"': 'End="0.005"',
+                                             "  </Simulation>": OUTPUTS.format(root=root).replace(
+                                                 'Every="0.001"', 'Every="0.002"')})
+    g = GrainsGPU.from_xml(path)
+    g.run()
+    lines = open(root + "_VelocityMaxMean.dat").read().splitlines()
+    times = [float(l.split("\t")[0]) for l in lines]
+    assert np.allclose(times, [0, 0.002, 0.004, 0.005], rtol=0, atol=1e-12)
+    assert lines[-1] == rawdata.velocity_line(g.time, *g.velocity_max_mean())
+    assert rawdata.read_rawdata(root, 100)["position_x"].shape == (3, 100)
