@@ -164,6 +164,7 @@ struct Part {
   V3 x, v, w, wb;  // centre, translational, space-fixed and body-fixed angular velocity
   Q q;
   V3 F, T, refp;   // torsor (Torsor.cpp:160-190)
+  double fsum = 0.;  // sum of the magnitudes of the contact forces applied (error normaliser, not a reference field)
   std::map<int, ContactMem> mem;  // contact memory by partner id (elementary ids are 1)
   double radius, crust, mass, inertia[6], req, density;
   int mat;
@@ -915,6 +916,8 @@ void forcesPP(System* s, HookeParams const& m, Part& p0, Part& p1, Contact const
     ContactMem e1 = hookeMemoryForces(m, bodyOf(p0), bodyOf(p1), ci, s->dt, existed, *e0, fn, ft, dm);
     p1.mem[p0.id] = e1;  // Component::addDeplContactInMap (:680-690)
     torsorAdd(p0.F, p0.T, p0.refp, ci.P, coef * (fn + ft));
+    p0.fsum += Norm(fn + ft);
+    p1.fsum += Norm(fn + ft);
     if (m.mur) p0.T += dm * coef;
     torsorAdd(p1.F, p1.T, p1.refp, ci.P, coef * (-fn - ft));
     if (m.mur) p1.T += -dm * coef;
@@ -922,6 +925,8 @@ void forcesPP(System* s, HookeParams const& m, Part& p0, Part& p1, Contact const
   }
   hookeForces(m, bodyOf(p0), bodyOf(p1), ci, fn, ft, dm);
   torsorAdd(p0.F, p0.T, p0.refp, ci.P, coef * (fn + ft));
+  p0.fsum += Norm(fn + ft);
+  p1.fsum += Norm(fn + ft);
   if (m.kr) p0.T += dm * coef;
   torsorAdd(p1.F, p1.T, p1.refp, ci.P, coef * (-fn - ft));
   if (m.kr) p1.T += -dm * coef;
@@ -943,6 +948,7 @@ void forcesPO(System* s, HookeParams const& m, Part& p0, Obs& o, Contact const& 
   const bool rolling = m.memory ? m.mur != 0. : m.kr != 0.;
   double coef = 1. / double(1);
   torsorAdd(p0.F, p0.T, p0.refp, ci.P, coef * (fn + ft));
+  p0.fsum += Norm(fn + ft);
   if (rolling) p0.T += dm * coef;
   if (!p0.clone) {
     torsorAdd(o.F, o.T, o.centre, ci.P, coef * (-fn - ft));
@@ -1085,6 +1091,7 @@ int computeParticlesForceAndTorque(System* s) {
     p.refp = p.x;
     p.F = p.weight;
     p.T = V3(0., 0., 0.);
+    p.fsum = 0.;
   }
   for (auto& o : s->obstacles) { o.F = V3(0., 0., 0.); o.T = V3(0., 0., 0.); }
   s->pairs.clear();
@@ -1879,6 +1886,14 @@ void orc_get_state(void* h, double* x, double* v, double* w, double* wb, double*
     }
     if (q) q[4 * i + 3] = p.q.w;
   }
+}
+
+// Per particle, the sum of the magnitudes of the contact forces of the last
+// force evaluation (the normaliser of force-error figures).
+void orc_get_force_sums(void* h, double* out) {
+  System* s = (System*)h;
+  int n = (int)s->insertion.size();
+  for (int id = 1; id <= n; ++id) out[id - 1] = s->pool[s->master_of_id[id]].fsum;
 }
 
 void orc_get_mass(void* h, double* mass, double* inertia, double* weight) {

@@ -44,6 +44,7 @@ def lib():
         L.orc_time.restype = ctypes.c_double
         L.orc_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_get_mass.argtypes = [vp, dp, dp, dp]
+        L.orc_get_force_sums.argtypes = [vp, dp]
         L.orc_get_obstacle_loads.argtypes = [vp, dp, dp]
         L.orc_get_pairs.argtypes = [vp, ip, ctypes.c_int]
         L.orc_get_grid.argtypes = [vp, ip, dp, dp]
@@ -228,6 +229,13 @@ class Oracle:
         m, I, wgt = np.zeros(n), np.zeros(n), np.zeros((n, 3))
         lib().orc_get_mass(self.h, _ptr(m), _ptr(I), _ptr(wgt))
         return m, I, wgt
+
+    def force_sums(self):
+        """Per particle, the sum of the magnitudes of the contact forces of the
+        last force evaluation (sum |f_contact|, the force-error normaliser)."""
+        out = np.zeros(self.n)
+        lib().orc_get_force_sums(self.h, _ptr(out))
+        return out
 
     def obstacle_loads(self):
         F, T = np.zeros((self.nobs, 3)), np.zeros((self.nobs, 3))

@@ -238,6 +238,48 @@ def test_newton_third_law_and_angular_momentum():
     assert np.max(np.abs(L.sum(axis=0))) < 1e-13 * lscale
 
 
+@pytest.mark.parametrize("w1", [0.0, 40.0])
+def test_rolling_resistance_moment(w1):
+    # HookeContactForceModel.cpp:95-111: M = -(kr Req |F_n| |w0 x (c0-P) -
+    # w1 x (c1-P)| / |w0 - w1|) (w0 - w1) on body 0, -M on body 1.  Two equal
+    # spheres meet head on along x, no gravity, frictionless (muc = 0), spun
+    # about z.  The normal force passes through both centres, so M is the only
+    # torque.  With the contact point half way (lever l = r - delta/2 on both
+    # sides) |w0 x (c0-P) - w1 x (c1-P)| = l |w0 + w1|, so w0 + w1 is conserved
+    # and d(w0 - w1)/dt = -2 kr Req l |F_n| (w0 + w1) / I: over the collision
+    # the spin difference drops by 2 kr Req r J (w0 + w1) / I up to the lever's
+    # relative excursion delta_max / 2r (~2e-3 here), with J the normal
+    # impulse, Req = r/2 and I = 2/5 m r^2.  A wrong Req, a dropped lever
+    # factor or a sign slip each miss by a factor.
+    kn, en, r, kr = 1e4, 0.9, 1e-3, 0.6
+    m = RHO * 4 / 3 * math.pi * r ** 3
+    inertia = 0.4 * m * r ** 2
+    mstar = m / 2
+    w0n = math.sqrt(kn / mstar)
+    tc = math.pi / math.sqrt(w0n ** 2 - (beta(en) * w0n) ** 2)
+    dt = tc / 2000
+    o = Oracle((0, 0, 0), (0.02, 0.02, 0.02), gravity=(0, 0, 0), dt=dt)
+    o.contact_model(0, 0, kn, en, 0.0, kr=kr)
+    u = 0.2
+    w0 = 100.0
+    o.add_spheres(r, RHO, 5e-5, 0, [[0.009 - 1e-6, 0.01, 0.01], [0.011, 0.01, 0.01]],
+                  [[u / 2, 0, 0], [-u / 2, 0, 0]], [[0, 0, w0], [0, 0, w1]])
+    o.init()
+    o.step(4000)
+    s = o.state()
+    assert len(o.pairs()) == 0
+    J = m * (u / 2 - s["v"][0, 0])  # impulse on body 0 (its velocity drops by J/m)
+    assert J == pytest.approx(mstar * (1 + en) * u, rel=3e-3)
+    drop = (w0 - w1) - (s["w"][0, 2] - s["w"][1, 2])
+    expected = 2 * kr * (r / 2) * r * J * (w0 + w1) / inertia
+    assert 0.2 < drop / (w0 - w1) < 0.9  # a sizeable, partial decay
+    assert drop == pytest.approx(expected, rel=3e-3)
+    # w0 + w1 is conserved; no spin appears about x or y; no tangential force
+    assert s["w"][0, 2] + s["w"][1, 2] == pytest.approx(w0 + w1, rel=1e-12)
+    assert np.max(np.abs(s["w"][:, :2])) == 0.0
+    assert np.max(np.abs(s["v"][:, 1:])) == 0.0
+
+
 def closest_point_box(c, L, centre, B):
     """independent sphere-box geometry: clamp the centre into the box frame"""
     B = np.asarray(B).reshape(3, 3)
