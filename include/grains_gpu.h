@@ -160,7 +160,8 @@ int grains_gpu_step_timed(grains_gpu_system* sys, int nsteps, double* ms);
  * on the system's stream: out[0] total ms, out[1] summed force-sweep ms,
  * out[2] number of force sweeps, out[3] summed binning ms, out[4] number of
  * binning sequences, out[5] summed neighbour-search ms, out[6] number of
- * neighbour searches.  No counterpart in the reference (instrumentation of
+ * neighbour searches, out[7] summed narrow-phase ms of non-spherical bodies,
+ * out[8] number of those narrow phases (out holds 9 doubles).  No counterpart in the reference (instrumentation of
  * the Grains::Simulation timers, solvercomputingtime.cpp:48-215). */
 int grains_gpu_profile_steps(grains_gpu_system* sys, int nsteps, double* out);
 
@@ -176,6 +177,11 @@ int grains_gpu_graph_stats(grains_gpu_system* sys, int* ngraphs, long* replays, 
 /* Forces and torques at the current state (Grains::computeParticlesForceAndTorque,
  * Grains.cpp:459-482); results via grains_gpu_get_state(F, T). */
 int grains_gpu_compute_forces(grains_gpu_system* sys);
+/* Forces at the current state, as grains_gpu_compute_forces, and for each
+ * particle (id order, n doubles) the sum of the magnitudes of the contact
+ * forces it received: the normaliser of per-particle force-error figures.
+ * No counterpart in the reference (a diagnostic of this library). */
+int grains_gpu_force_sums(grains_gpu_system* sys, double* out);
 
 /* State in id order: x, v, w, wb, F, T: 3 doubles per particle; q: 4 (x, y,
  * z, w).  NULL pointers are skipped. */

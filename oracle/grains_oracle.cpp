@@ -165,6 +165,7 @@ struct Part {
   Q q;
   V3 F, T, refp;   // torsor (Torsor.cpp:160-190)
   double fsum = 0.;  // sum of the magnitudes of the contact forces applied (error normaliser, not a reference field)
+  double tsum = 0.;  // sum of the term scales of hookeForces (error normaliser, not a reference field)
   std::map<int, ContactMem> mem;  // contact memory by partner id (elementary ids are 1)
   double radius, crust, mass, inertia[6], req, density;
   int mat;
@@ -749,8 +750,11 @@ struct Body {  // what performForcesCalculus reads from a Component
 };
 inline V3 velAt(Body const& b, V3 const& P) { return b.v + cross(b.w, P - b.x); }  // Particle.cpp:661-665, ParticleKinematics.cpp:229-232
 
+// termscale (optional, not a reference quantity): kn |pen| + (|gamman| +
+// |gammat|) |v_rel|, the magnitude of the terms the force is made of, the
+// scale that the rounding of a reduced-precision evaluation is relative to.
 void hookeForces(HookeParams const& m, Body const& p0, Body const& p1, Contact const& ci, V3& delFN, V3& delFT,
-                 V3& delM) {
+                 V3& delM, double* termscale = nullptr) {
   V3 P = ci.P;
   V3 pen = ci.ov;
   V3 normal(pen);
@@ -767,6 +771,7 @@ void hookeForces(HookeParams const& m, Body const& p0, Body const& p1, Contact c
   double etat = m.etat == -1. ? -m.beta * std::sqrt(m.kn / avmass) : m.etat;
   double gammat = 2. * etat * avmass;
   delFT = -gammat * v_t;
+  if (termscale) *termscale = m.kn * Norm(pen) + (std::fabs(gamman) + std::fabs(gammat)) * Norm(tmpV);
   double fn = m.muc * normFN;
   double ft = Norm(delFT);
   if (fn < ft) {
@@ -923,10 +928,13 @@ void forcesPP(System* s, HookeParams const& m, Part& p0, Part& p1, Contact const
     if (m.mur) p1.T += -dm * coef;
     return;
   }
-  hookeForces(m, bodyOf(p0), bodyOf(p1), ci, fn, ft, dm);
+  double ts = 0.;
+  hookeForces(m, bodyOf(p0), bodyOf(p1), ci, fn, ft, dm, &ts);
   torsorAdd(p0.F, p0.T, p0.refp, ci.P, coef * (fn + ft));
   p0.fsum += Norm(fn + ft);
   p1.fsum += Norm(fn + ft);
+  p0.tsum += ts;
+  p1.tsum += ts;
   if (m.kr) p0.T += dm * coef;
   torsorAdd(p1.F, p1.T, p1.refp, ci.P, coef * (-fn - ft));
   if (m.kr) p1.T += -dm * coef;
@@ -943,7 +951,9 @@ void forcesPO(System* s, HookeParams const& m, Part& p0, Obs& o, Contact const& 
     bool existed = getContactMemory(p0, o.id, e0);
     hookeMemoryForces(m, bodyOf(p0), b1, ci, s->dt, existed, *e0, fn, ft, dm);
   } else {
-    hookeForces(m, bodyOf(p0), b1, ci, fn, ft, dm);
+    double ts = 0.;
+    hookeForces(m, bodyOf(p0), b1, ci, fn, ft, dm, &ts);
+    p0.tsum += ts;
   }
   const bool rolling = m.memory ? m.mur != 0. : m.kr != 0.;
   double coef = 1. / double(1);
@@ -1092,6 +1102,7 @@ int computeParticlesForceAndTorque(System* s) {
     p.F = p.weight;
     p.T = V3(0., 0., 0.);
     p.fsum = 0.;
+    p.tsum = 0.;
   }
   for (auto& o : s->obstacles) { o.F = V3(0., 0., 0.); o.T = V3(0., 0., 0.); }
   s->pairs.clear();
@@ -1894,6 +1905,14 @@ void orc_get_force_sums(void* h, double* out) {
   System* s = (System*)h;
   int n = (int)s->insertion.size();
   for (int id = 1; id <= n; ++id) out[id - 1] = s->pool[s->master_of_id[id]].fsum;
+}
+
+// Per particle, the sum of the term scales of its memoryless contact forces
+// in the last force evaluation (kn |pen| + (|gamman| + |gammat|) |v_rel|).
+void orc_get_term_scales(void* h, double* out) {
+  System* s = (System*)h;
+  int n = (int)s->insertion.size();
+  for (int id = 1; id <= n; ++id) out[id - 1] = s->pool[s->master_of_id[id]].tsum;
 }
 
 void orc_get_mass(void* h, double* mass, double* inertia, double* weight) {

@@ -45,6 +45,7 @@ def lib():
         L.orc_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_get_mass.argtypes = [vp, dp, dp, dp]
         L.orc_get_force_sums.argtypes = [vp, dp]
+        L.orc_get_term_scales.argtypes = [vp, dp]
         L.orc_get_obstacle_loads.argtypes = [vp, dp, dp]
         L.orc_get_pairs.argtypes = [vp, ip, ctypes.c_int]
         L.orc_get_grid.argtypes = [vp, ip, dp, dp]
@@ -235,6 +236,14 @@ class Oracle:
         last force evaluation (sum |f_contact|, the force-error normaliser)."""
         out = np.zeros(self.n)
         lib().orc_get_force_sums(self.h, _ptr(out))
+        return out
+
+    def term_scales(self):
+        """Per particle, the sum over its memoryless contacts of kn |pen| +
+        (|gamman| + |gammat|) |v_rel|: the magnitude of the terms its contact
+        forces are made of (the scale of reduced-precision rounding)."""
+        out = np.zeros(self.n)
+        lib().orc_get_term_scales(self.h, _ptr(out))
         return out
 
     def obstacle_loads(self):

@@ -16,6 +16,8 @@
 // start.  Both return the maximising vertex; they can differ only among tied
 // maxima, i.e. for a face exactly parallel to the search direction.
 #pragma once
+#include <type_traits>
+
 #include "dem_math.hpp"
 #include "dem_types.hpp"
 
@@ -95,102 +97,187 @@ DEM_HD d3 support(const ConvexGeo& g, d3 v) {
 }
 
 // --- Johnson sub-distance algorithm -----------------------------------------
+// The simplex of up to four points of the Minkowski difference and the
+// determinant cache of Convex.cpp:219-435, with the same arithmetic in the
+// same order.  Every table index below is a compile-time constant -- the
+// slot of the point being added is a template argument, the subsets are
+// enumerated by unrolled loops -- so that the tables live in registers; the
+// run-time masks only pick which of the constant-indexed branches runs.
+// Only the 32 entries det[s][i] with i in s and the upper triangle of the
+// symmetric dot-product table are ever read.
 struct Simplex {
   d3 y[4], p[4], q[4];
   double det[16][4];
-  double dp[4][4];
+  double dp[10];  // y_i . y_j for i <= j
 };
 
-DEM_HD void j_det(Simplex& S, unsigned bits, unsigned last, unsigned last_bit, unsigned all_bits) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if (bits & bit) S.dp[i][last] = S.dp[last][i] = dot(S.y[i], S.y[last]);
-  S.dp[last][last] = dot(S.y[last], S.y[last]);
-  S.det[last_bit][last] = 1.;
-  for (unsigned j = 0, sj = 1; j < 4; ++j, sj <<= 1) {
-    if (!(bits & sj)) continue;
-    unsigned s2 = sj | last_bit;
-    S.det[s2][j] = S.dp[last][last] - S.dp[last][j];
-    S.det[s2][last] = S.dp[j][j] - S.dp[j][last];
-    for (unsigned k = 0, sk = 1; k < j; ++k, sk <<= 1) {
-      if (!(bits & sk)) continue;
-      unsigned s3 = sk | s2;
-      S.det[s3][k] = S.det[s2][j] * (S.dp[j][j] - S.dp[j][k]) + S.det[s2][last] * (S.dp[last][j] - S.dp[last][k]);
-      S.det[s3][j] = S.det[sk | last_bit][k] * (S.dp[k][k] - S.dp[k][j]) +
-                     S.det[sk | last_bit][last] * (S.dp[last][k] - S.dp[last][j]);
-      S.det[s3][last] =
-          S.det[sk | sj][k] * (S.dp[k][k] - S.dp[k][last]) + S.det[sk | sj][j] * (S.dp[j][k] - S.dp[j][last]);
-    }
+// index of y_i . y_j in Simplex::dp, a constant expression (the tables are
+// addressed at compile time only)
+template <int I, int J>
+constexpr int kDp = I <= J ? I * 4 - I * (I - 1) / 2 + (J - I) : J * 4 - J * (J - 1) / 2 + (I - J);
+
+// f(integral_constant<int, i>) for i = B, B+1, ..., E-1
+template <int B, int E, class F>
+DEM_HD void sfor(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    sfor<B + 1, E>(f);
   }
+}
+// f(integral_constant<int, i>) for i = B, B-1, ..., E+1
+template <int B, int E, class F>
+DEM_HD void sfor_down(F&& f) {
+  if constexpr (B > E) {
+    f(std::integral_constant<int, B>{});
+    sfor_down<B - 1, E>(f);
+  }
+}
+
+// computeDet (Convex.cpp:219-268) for the point just added in slot LAST
+template <int LAST>
+DEM_HD void j_det(Simplex& S, unsigned bits, unsigned all_bits) {
+  constexpr unsigned last_bit = 1u << LAST;
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (i != LAST)
+      if (bits & (1u << i)) S.dp[kDp<i, LAST>] = dot(S.y[i], S.y[LAST]);
+  });
+  S.dp[kDp<LAST, LAST>] = dot(S.y[LAST], S.y[LAST]);
+  S.det[last_bit][LAST] = 1.;
+  sfor<0, 4>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if constexpr (j != LAST) {
+      constexpr unsigned sj = 1u << j, s2 = sj | last_bit;
+      if (bits & sj) {
+        S.det[s2][j] = S.dp[kDp<LAST, LAST>] - S.dp[kDp<LAST, j>];
+        S.det[s2][LAST] = S.dp[kDp<j, j>] - S.dp[kDp<j, LAST>];
+        sfor<0, j>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          if constexpr (k != LAST) {
+            constexpr unsigned sk = 1u << k, s3 = sk | s2;
+            if (bits & sk) {
+              S.det[s3][k] = S.det[s2][j] * (S.dp[kDp<j, j>] - S.dp[kDp<j, k>]) +
+                             S.det[s2][LAST] * (S.dp[kDp<LAST, j>] - S.dp[kDp<LAST, k>]);
+              S.det[s3][j] = S.det[sk | last_bit][k] * (S.dp[kDp<k, k>] - S.dp[kDp<k, j>]) +
+                             S.det[sk | last_bit][LAST] * (S.dp[kDp<LAST, k>] - S.dp[kDp<LAST, j>]);
+              S.det[s3][LAST] = S.det[sk | sj][k] * (S.dp[kDp<k, k>] - S.dp[kDp<k, LAST>]) +
+                                S.det[sk | sj][j] * (S.dp[kDp<j, k>] - S.dp[kDp<j, LAST>]);
+            }
+          }
+        });
+      }
+    }
+  });
   if (all_bits == 15) {
-    S.det[15][0] = S.det[14][1] * (S.dp[1][1] - S.dp[1][0]) + S.det[14][2] * (S.dp[2][1] - S.dp[2][0]) +
-                   S.det[14][3] * (S.dp[3][1] - S.dp[3][0]);
-    S.det[15][1] = S.det[13][0] * (S.dp[0][0] - S.dp[0][1]) + S.det[13][2] * (S.dp[2][0] - S.dp[2][1]) +
-                   S.det[13][3] * (S.dp[3][0] - S.dp[3][1]);
-    S.det[15][2] = S.det[11][0] * (S.dp[0][0] - S.dp[0][2]) + S.det[11][1] * (S.dp[1][0] - S.dp[1][2]) +
-                   S.det[11][3] * (S.dp[3][0] - S.dp[3][2]);
-    S.det[15][3] = S.det[7][0] * (S.dp[0][0] - S.dp[0][3]) + S.det[7][1] * (S.dp[1][0] - S.dp[1][3]) +
-                   S.det[7][2] * (S.dp[2][0] - S.dp[2][3]);
+    const double* dp = S.dp;
+    S.det[15][0] = S.det[14][1] * (dp[kDp<1, 1>] - dp[kDp<1, 0>]) + S.det[14][2] * (dp[kDp<2, 1>] - dp[kDp<2, 0>]) +
+                   S.det[14][3] * (dp[kDp<3, 1>] - dp[kDp<3, 0>]);
+    S.det[15][1] = S.det[13][0] * (dp[kDp<0, 0>] - dp[kDp<0, 1>]) + S.det[13][2] * (dp[kDp<2, 0>] - dp[kDp<2, 1>]) +
+                   S.det[13][3] * (dp[kDp<3, 0>] - dp[kDp<3, 1>]);
+    S.det[15][2] = S.det[11][0] * (dp[kDp<0, 0>] - dp[kDp<0, 2>]) + S.det[11][1] * (dp[kDp<1, 0>] - dp[kDp<1, 2>]) +
+                   S.det[11][3] * (dp[kDp<3, 0>] - dp[kDp<3, 2>]);
+    S.det[15][3] = S.det[7][0] * (dp[kDp<0, 0>] - dp[kDp<0, 3>]) + S.det[7][1] * (dp[kDp<1, 0>] - dp[kDp<1, 3>]) +
+                   S.det[7][2] * (dp[kDp<2, 0>] - dp[kDp<2, 3>]);
   }
 }
-DEM_HD bool j_valid(const Simplex& S, unsigned s, unsigned all_bits) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1) {
-    if (!(all_bits & bit)) continue;
-    if (s & bit) {
-      if (S.det[s][i] <= kEps3) return false;
-    } else if (S.det[s | bit][i] > 0.) {
-      return false;
+// valid (Convex.cpp:272-290) for the subset SS
+template <unsigned SS>
+DEM_HD bool j_valid(const Simplex& S, unsigned all_bits) {
+  bool ok = true;
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr unsigned bit = 1u << i;
+    if (!ok || !(all_bits & bit)) return;
+    if constexpr ((SS & bit) != 0u) {
+      if (S.det[SS][i] <= kEps3) ok = false;
+    } else {
+      if (S.det[SS | bit][i] > 0.) ok = false;
     }
-  }
-  return true;
+  });
+  return ok;
 }
-DEM_HD d3 j_vector(const Simplex& S, unsigned bits) {
+// computeVector (Convex.cpp:294-306)
+template <unsigned SS>
+DEM_HD d3 j_vector(const Simplex& S) {
   double sum = 0.;
   d3 v{0., 0., 0.};
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if (bits & bit) {
-      sum += S.det[bits][i];
-      v += S.det[bits][i] * S.y[i];
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr ((SS & (1u << i)) != 0u) {
+      sum += S.det[SS][i];
+      v += S.det[SS][i] * S.y[i];
     }
+  });
   v *= 1. / sum;
   return v;
 }
-DEM_HD void j_points(const Simplex& S, unsigned bits, d3& p1, d3& p2) {
+// computePoints (Convex.cpp:310-327)
+template <unsigned SS>
+DEM_HD void j_points_t(const Simplex& S, d3& p1, d3& p2) {
   double sum = 0.;
   p1 = d3{0., 0., 0.};
   p2 = d3{0., 0., 0.};
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if (bits & bit) {
-      sum += S.det[bits][i];
-      p1 += S.det[bits][i] * S.p[i];
-      p2 += S.det[bits][i] * S.q[i];
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr ((SS & (1u << i)) != 0u) {
+      sum += S.det[SS][i];
+      p1 += S.det[SS][i] * S.p[i];
+      p2 += S.det[SS][i] * S.q[i];
     }
+  });
   double s = 1. / sum;
   p1 *= s;
   p2 *= s;
 }
-DEM_HD bool j_proper(const Simplex& S, unsigned s) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if ((s & bit) && S.det[s][i] <= kEps3) return false;
-  return true;
-}
-DEM_HD bool j_closest(Simplex& S, unsigned& bits, unsigned last, unsigned last_bit, unsigned all_bits, d3& v) {
-  j_det(S, bits, last, last_bit, all_bits);
-  for (unsigned s = bits; s; --s) {
-    if ((s & bits) == s && j_valid(S, s | last_bit, all_bits)) {
-      bits = s | last_bit;
-      v = j_vector(S, bits);
-      return true;
+DEM_HD void j_points(const Simplex& S, unsigned bits, d3& p1, d3& p2) {
+  bool done = false;
+  sfor_down<15, -1>([&](auto B) {
+    constexpr unsigned b = (unsigned)decltype(B)::value;
+    if (!done && bits == b) {
+      j_points_t<b>(S, p1, p2);
+      done = true;
     }
-  }
-  if (j_valid(S, last_bit, all_bits)) {
+  });
+}
+// proper (Convex.cpp:331-338)
+template <unsigned SS>
+DEM_HD bool j_proper(const Simplex& S) {
+  bool ok = true;
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr ((SS & (1u << i)) != 0u)
+      if (S.det[SS][i] <= kEps3) ok = false;
+  });
+  return ok;
+}
+// closest (Convex.cpp:342-414): the smallest valid subset containing LAST,
+// else the backup procedure over every proper subset
+template <int LAST>
+DEM_HD bool j_closest(Simplex& S, unsigned& bits, unsigned all_bits, d3& v) {
+  constexpr unsigned last_bit = 1u << LAST;
+  j_det<LAST>(S, bits, all_bits);
+  bool found = false;
+  sfor_down<15, 0>([&](auto SI) {
+    constexpr unsigned s = (unsigned)decltype(SI)::value;
+    if constexpr ((s & last_bit) == 0u) {
+      if (!found && (s & bits) == s && j_valid<s | last_bit>(S, all_bits)) {
+        bits = s | last_bit;
+        v = j_vector<s | last_bit>(S);
+        found = true;
+      }
+    }
+  });
+  if (found) return true;
+  if (j_valid<last_bit>(S, all_bits)) {
     bits = last_bit;
-    v = S.y[last];
+    v = S.y[LAST];
     return true;
   }
   double min_dist2 = INFINITY;
-  for (unsigned s = all_bits; s; --s) {
-    if ((s & all_bits) == s && j_proper(S, s)) {
-      d3 u = j_vector(S, s);
+  sfor_down<15, 0>([&](auto SI) {
+    constexpr unsigned s = (unsigned)decltype(SI)::value;
+    if ((s & all_bits) == s && j_proper<s>(S)) {
+      d3 u = j_vector<s>(S);
       double d2 = dot(u, u);
       if (d2 < min_dist2) {
         min_dist2 = d2;
@@ -198,50 +285,62 @@ DEM_HD bool j_closest(Simplex& S, unsigned& bits, unsigned last, unsigned last_b
         v = u;
       }
     }
-  }
+  });
   return false;
 }
+// degenerate (Convex.cpp:418-435)
 DEM_HD bool j_degenerate(const Simplex& S, unsigned all_bits, d3 w) {
-  for (unsigned i = 0, bit = 1; i < 4; ++i, bit <<= 1)
-    if ((all_bits & bit) && S.y[i].x == w.x && S.y[i].y == w.y && S.y[i].z == w.z) return true;
-  return false;
+  bool deg = false;
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if ((all_bits & (1u << i)) && S.y[i].x == w.x && S.y[i].y == w.y && S.y[i].z == w.z) deg = true;
+  });
+  return deg;
 }
 
-// closest_points with relative tolerance 1e-10; witness points in the local
-// frames; returns the distance
+// One iteration of closest_points with the new point in slot LAST: 1 to go
+// on, 0 when the loop ends.
+template <int LAST>
+DEM_HD int gjk_iter(Simplex& S, const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w,
+                    unsigned& bits, unsigned& all_bits, d3& v, double& dist) {
+  constexpr unsigned last_bit = 1u << LAST;
+  const double relError = kEps, absError = 1.e-4 * relError;
+  d3 d = v;
+  S.p[LAST] = support(A, tf_vec_basis(-d, a2w));
+  S.q[LAST] = support(B, tf_vec_basis(d, b2w));
+  d3 w = tf_apply(a2w, S.p[LAST]) - tf_apply(b2w, S.q[LAST]);
+  double mu = dist - dot(v, w) / dist;
+  if (mu < dist * relError || mu < absError) return 0;
+  if (j_degenerate(S, all_bits, w)) return 0;
+  S.y[LAST] = w;
+  all_bits = bits | last_bit;
+  if (!j_closest<LAST>(S, bits, all_bits, v)) return 0;
+  dist = norm(v);
+  return 1;
+}
+
+// closest_points with relative tolerance 1e-10 (Convex.cpp:536-627);
+// witness points in the local frames; returns the distance
 DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, d3& pa, d3& pb,
                           int& iters) {
   Simplex S;
   for (int i = 0; i < 16; ++i)
     for (int j = 0; j < 4; ++j) S.det[i][j] = 0.;
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) S.dp[i][j] = 0.;
-  unsigned bits = 0, last = 0, last_bit = 0, all_bits = 0;
-  const double relError = kEps, absError = 1.e-4 * relError;
+  for (int i = 0; i < 10; ++i) S.dp[i] = 0.;
+  unsigned bits = 0, all_bits = 0;
   d3 zero{0., 0., 0.};
   d3 v = tf_apply(a2w, support(A, zero)) - tf_apply(b2w, support(B, zero));
-  d3 w = v;
   double dist = norm(v);
   int it = 0;
   while (bits < 15 && dist > kEps2 && it < 1000) {
     ++it;
-    last = 0;
-    last_bit = 1;
-    while (bits & last_bit) {
-      ++last;
-      last_bit <<= 1;
-    }
-    d3 d = v;
-    S.p[last] = support(A, tf_vec_basis(-d, a2w));
-    S.q[last] = support(B, tf_vec_basis(d, b2w));
-    w = tf_apply(a2w, S.p[last]) - tf_apply(b2w, S.q[last]);
-    double mu = dist - dot(v, w) / dist;
-    if (mu < dist * relError || mu < absError) break;
-    if (j_degenerate(S, all_bits, w)) break;
-    S.y[last] = w;
-    all_bits = bits | last_bit;
-    if (!j_closest(S, bits, last, last_bit, all_bits, v)) break;
-    dist = norm(v);
+    // the first free slot
+    int go;
+    if (!(bits & 1u)) go = gjk_iter<0>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
+    else if (!(bits & 2u)) go = gjk_iter<1>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
+    else if (!(bits & 4u)) go = gjk_iter<2>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
+    else go = gjk_iter<3>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
+    if (!go) break;
   }
   j_points(S, bits, pa, pb);
   iters = it;
@@ -299,6 +398,56 @@ DEM_HD bool recover_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, d
   ov = imposed * ba;
   d = -imposed;
   return true;
+}
+
+// One attempt of the general ClosestPoint path: attempt 0 with the
+// crust-shrunk transforms (closest_gjk), attempt 1 with the bodies shrunk by
+// ten crusts (recover_gjk, the ContactError recovery).  Returns 1 with the
+// contact features (d may be >= 0), 2 with the recovered features, 0 when
+// the circumscribed spheres do not meet (attempt 0), -1 when the shrunk
+// bodies intersect (attempt 0: go on with attempt 1; attempt 1:
+// ContactError).  A caller that evaluates every pair through one call of
+// this function holds a single copy of the GJK state in registers.
+DEM_HD int gjk_attempt(const ConvexGeo& A, const ConvexGeo& B, int attempt, d3& P, d3& ov, double& d,
+                       int& iters) {
+  d3 sa = A.scaling, sb = B.scaling;
+  if (attempt == 0) {
+    d3 gcagcb = A.o - B.o;
+    if (!(norm(gcagcb) < A.circR + B.circR)) return 0;
+  } else {
+    double* as[3] = {&sa.x, &sa.y, &sa.z};
+    double* bs[3] = {&sb.x, &sb.y, &sb.z};
+    const double ae[3] = {A.bext.x, A.bext.y, A.bext.z}, be[3] = {B.bext.x, B.bext.y, B.bext.z};
+    for (int a = 0; a < 3; ++a) {
+      double x = ae[a] < kEps ? 1. : (ae[a] - 10. * A.crust) / ae[a];
+      *as[a] = x > 0.5 ? x : 0.5;
+      double y = be[a] < kEps ? 1. : (be[a] - 10. * B.crust) / be[a];
+      *bs[a] = y > 0.5 ? y : 0.5;
+    }
+  }
+  const double crusts = A.crust + B.crust;
+  Tf a2w = tf_scaled(A, sa), b2w = tf_scaled(B, sb);
+  d3 pa, pb;
+  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, iters);
+  if (distance < kEps) return -1;
+  pa = tf_apply(a2w, pa);
+  pb = tf_apply(b2w, pb);
+  d3 ba = pa - pb;
+  if (attempt == 0) {
+    P = pa / 2.0 + pb / 2.0;
+    d3 o = ba / distance;
+    round_small(o);
+    o *= crusts - distance;
+    ov = o;
+    d = distance - crusts;
+    return 1;
+  }
+  ba /= norm(ba);
+  P = pa / 2.0 + pb / 2.0;
+  double imposed = 1. * crusts;
+  ov = imposed * ba;
+  d = -imposed;
+  return 2;
 }
 
 // --- kinematics of non-spherical bodies --------------------------------------

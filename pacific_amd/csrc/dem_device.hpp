@@ -408,16 +408,24 @@ DEM_HD bool hooke_memory64(const HookeModel& m, d3 x0, d3 v0, d3 w0, double m0, 
   return slip;
 }
 
-// Same model in FP32 on lever arms relative to each body's centre.
-DEM_HD bool hooke32(const HookeModel& m, f3 l0, f3 v0, f3 w0, float m0, float req0, f3 l1, f3 v1, f3 w1,
-                    float m1, float req1, f3 pen, f3& f, f3& dM) {
+// Velocity of body 0 relative to body 1 at the contact point P
+// (Particle.cpp:661-665), in FP64: the translational velocities of two
+// bodies in contact are close, and their difference in FP32 would lose the
+// relative velocity the dashpot acts on.
+DEM_HD d3 rel_velocity(d3 x0, d3 v0, d3 w0, d3 x1, d3 v1, d3 w1, d3 P) {
+  return (v0 + cross(w0, P - x0)) - (v1 + cross(w1, P - x1));
+}
+
+// Mixed precision: the force arithmetic in FP32 on lever arms l = P - x and
+// the FP64 relative velocity tmpV at the contact point.
+DEM_HD bool hooke32(const HookeModel& m, f3 l0, f3 w0, float m0, float req0, f3 l1, f3 w1, float m1, float req1,
+                    f3 tmpV, f3 pen, f3& f, f3& dM) {
   const float kn = (float)m.kn, beta = (float)m.beta;
   float npen = normf(pen);
   f3 normal = pen * (1.f / npen);
   normal.x = fabsf(normal.x) < 1e-10f ? 0.f : normal.x;
   normal.y = fabsf(normal.y) < 1e-10f ? 0.f : normal.y;
   normal.z = fabsf(normal.z) < 1e-10f ? 0.f : normal.z;
-  f3 tmpV = (v0 + crossf(w0, l0)) - (v1 + crossf(w1, l1));
   f3 v_n = normal * dotf(tmpV, normal);
   f3 v_t = tmpV - v_n;
   float avmass = 1.f / (1.f / m0 + 1.f / m1);
@@ -1123,6 +1131,7 @@ struct ForceVisitor {
   f3 Fc, Tc;
   int s = -1;     // storage index of self (contact memory)
   int nmem = 0;   // memory entries written by this evaluation
+  double fs = 0.; // sum of the contact-force magnitudes (EXTRAS with sp.fsum)
 
   // HookeMemory: this pair's entry of the last evaluation (zero when new),
   // the force, and this evaluation's entry.  Evaluated in FP64 in both
@@ -1165,6 +1174,7 @@ struct ForceVisitor {
     F += f;
     T += cross(Pc - me.x, f);
     if (hm.mur != 0.) T += dM * 1.;
+    if (sp.fsum) fs += norm(f);
     return slip;
   }
 
@@ -1181,17 +1191,19 @@ struct ForceVisitor {
     } else if (MIXED) {
       f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - xo);
       f3 f, dM;
-      slip = hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3of(vo), f3of(wo), (float)mo,
-                     (float)reqo, f3of(ov), f, dM);
+      slip = hooke32(hm, l0, f3of(me.w), (float)me.m, (float)me.req, l1, f3of(wo), (float)mo, (float)reqo,
+                     f3of(rel_velocity(me.x, me.v, me.w, xo, vo, wo, Pc)), f3of(ov), f, dM);
       Fc += f;
       Tc += crossf(l0, f);
       if (hm.kr != 0.) Tc += dM;
+      if (EXTRAS && sp.fsum) fs += (double)normf(f);
     } else {
       d3 f, dM;
       slip = hooke64(hm, me.x, me.v, me.w, me.m, me.req, xo, vo, wo, mo, reqo, Pc, ov, f, dM);
       F += f;
       T += cross(Pc - me.x, f);
       if (hm.kr != 0.) T += dM * 1.;
+      if (EXTRAS && sp.fsum) fs += norm(f);
     }
     record_pair(sp, st, pairs, me.id, pid2, slip);
   }
@@ -1208,17 +1220,19 @@ struct ForceVisitor {
     } else if (MIXED) {
       f3 l0 = f3of(Pc - me.x), l1 = f3of(Pc - cen);
       f3 f, dM;
-      slip = hooke32(hm, l0, f3of(me.v), f3of(me.w), (float)me.m, (float)me.req, l1, f3{0.f, 0.f, 0.f},
-                     f3{0.f, 0.f, 0.f}, (float)kObstacleMass, (float)B.req, f3of(ov), f, dM);
+      slip = hooke32(hm, l0, f3of(me.w), (float)me.m, (float)me.req, l1, f3{0.f, 0.f, 0.f}, (float)kObstacleMass,
+                     (float)B.req, f3of(rel_velocity(me.x, me.v, me.w, cen, zero, zero, Pc)), f3of(ov), f, dM);
       Fc += f;
       Tc += crossf(l0, f);
       if (hm.kr != 0.) Tc += dM;
+      if (EXTRAS && sp.fsum) fs += (double)normf(f);
     } else {
       d3 f, dM;
       slip = hooke64(hm, me.x, me.v, me.w, me.m, me.req, cen, zero, zero, kObstacleMass, B.req, Pc, ov, f, dM);
       F += f;
       T += cross(Pc - me.x, f);
       if (hm.kr != 0.) T += dM * 1.;
+      if (EXTRAS && sp.fsum) fs += norm(f);
     }
     record_pair(sp, st, pairs, me.id, B.id, slip);
   }
@@ -1249,6 +1263,7 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
     F += d3{(double)vis.Fc.x, (double)vis.Fc.y, (double)vis.Fc.z};
     T += d3{(double)vis.Tc.x, (double)vis.Tc.y, (double)vis.Tc.z};
   }
+  if (EXTRAS && sp.fsum && s >= 0) sp.fsum[s] = vis.fs;
   // this evaluation's contact memory of the particle (every particle, so
   // that entries of contacts that ended are dropped)
   if (EXTRAS && sp.mem.cap && s >= 0) {

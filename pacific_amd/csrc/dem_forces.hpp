@@ -65,6 +65,42 @@ __device__ __forceinline__ void sphere_tail(const Particles& P, const Sorted& S,
   P.q[3][s] = kin.q.w;
 }
 
+// Force and torque of one body from the contacts the narrow phase recorded
+// for its sorted slot k (ContactFeat), in the order it met them -- the order
+// of the 27-cell sweep -- so that the sum is the one of an inline sweep.
+template <bool MIXED, bool EXTRAS>
+__device__ __forceinline__ void listed_contacts(const Particles& P, const Sorted& S, const BoxObstacle* __restrict__ obs,
+                                                const HookeModel* __restrict__ models, const StepParams& sp,
+                                                DevStatus* st, int* pairs, int cv, int k, int s, const Self& me, d3& F,
+                                                d3& T) {
+  using FV = ForceVisitor<MIXED, EXTRAS>;
+  FV vis{P, S, obs, models, sp, st, pairs, cv, me, F, T, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}};
+  vis.s = s;
+  const ContactFeat& cf = sp.cf;
+  const int c = min(cf.count[k], cf.cap);
+  for (int j = 0; j < c; ++j) {
+    const size_t e = (size_t)j * cf.n + k;
+    const int o = cf.ent[e];
+    const d3 Pc{cf.f[3][e], cf.f[4][e], cf.f[5][e]}, ov{cf.f[6][e], cf.f[7][e], cf.f[8][e]};
+    if (o >= 0) {
+      if (o < sp.n) vis.pair(-1, o, cf.pid[e], d3{cf.f[0][e], cf.f[1][e], cf.f[2][e]}, 0., Pc, ov, -1.);
+    } else if (-1 - o < sp.nobs) {
+      vis.wall(-1 - o, Pc, ov, -1.);
+    }
+  }
+  F = vis.F;
+  T = vis.T;
+  if (MIXED) {
+    F += d3{(double)vis.Fc.x, (double)vis.Fc.y, (double)vis.Fc.z};
+    T += d3{(double)vis.Tc.x, (double)vis.Tc.y, (double)vis.Tc.z};
+  }
+  if (EXTRAS && sp.fsum) sp.fsum[s] = vis.fs;
+  if (EXTRAS && sp.mem.cap) {
+    sp.mem.n[sp.mem.cur ^ 1][s] = min(vis.nmem, sp.mem.cap);
+    if (vis.nmem > sp.mem.cap) raise_error(st, kMemoryOverflow, me.id, vis.nmem);
+  }
+}
+
 template <int MODE, bool MIXED, bool SHAPES, bool EXTRAS>
 __device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, const Grid& G,
                                           const BoxObstacle* __restrict__ obs, const HookeModel* __restrict__ models,
@@ -98,10 +134,19 @@ __device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, c
   d3 F{sp.g[0] * wf, sp.g[1] * wf, sp.g[2] * wf};
   d3 T{0., 0., 0.};
 
-  ConvexGeo mg;
-  if (SHAPES) mg = convex_of(P, s, me.x, cx);
-  sweep_particle<MIXED, SHAPES, EXTRAS>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T, &mg,
-                                cx, &L, &VS, s, &CL);
+  // non-spherical bodies: the basis (drift) and whether the body rotates
+  // with the gyroscopic term; their contacts come from the narrow phase
+  double B[SHAPES ? 9 : 1];
+  bool gyro = false;
+  if (SHAPES) {
+    for (int j = 0; j < 9; ++j) B[j] = P.B[cx][j][s];
+    const int sh = P.shape[s];
+    gyro = sh >= 0 && P.shapes[sh].type != kShapeSphere;
+    listed_contacts<MIXED, EXTRAS>(P, S, obs, models, sp, st, pairs, cv, k, s, me, F, T);
+  } else {
+    sweep_particle<MIXED, false, EXTRAS>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T,
+                                         nullptr, cx, &L, &VS, s, &CL);
+  }
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
   if (!(MODE == kFused && sp.scheme == kLeapFrog)) {
@@ -119,15 +164,14 @@ __device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, c
   const double cf = P.cfac[s], I = P.inertia[s];
   const int nv = cv ^ 1;
   if (SHAPES) {
-    const bool gyro = mg.type != kShapeSphere;
     const d3 I3 = gyro ? d3{P.I3[0][s], P.I3[1][s], P.I3[2][s]} : d3{I, I, I};
     lf_kick_body(kin, gyro, F, T, me.m, cf, I, I3, 0.5 * sp.dt);
     if (MODE == kFused) {
-      double depl = lf_move_body(kin, gyro, F, T, me.m, cf, I, I3, 0.5 * sp.dt, sp.dt, mg.B);
+      double depl = lf_move_body(kin, gyro, F, T, me.m, cf, I, I3, 0.5 * sp.dt, sp.dt, B);
       if (depl > me.c) raise_error(st, kMotionError, me.id, 0);
       wrap_and_bin(kin, G, S, st, s, me.id, 0, &VS);
       st3(P.x[cx ^ 1], s, kin.x);
-      for (int j = 0; j < 9; ++j) P.B[cx ^ 1][j][s] = mg.B[j];
+      for (int j = 0; j < 9; ++j) P.B[cx ^ 1][j][s] = B[j];
       P.q[0][s] = kin.q.v.x;
       P.q[1][s] = kin.q.v.y;
       P.q[2][s] = kin.q.v.z;
@@ -143,6 +187,9 @@ __device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, c
 
 // Per-block sums of the contact statistics (sum and max of the overlaps,
 // their count, GJK iterations and evaluations), in a fixed order.
+// (ADD: combined with the sums already in out, written by an earlier launch
+// over the same particles.)
+template <bool ADD = false>
 __device__ __forceinline__ void block_stats(const StatAcc& a, double* out) {
   __shared__ double part[4][5];
   double v[5] = {a.sum, a.max, a.cnt, a.git, a.gn};
@@ -162,6 +209,7 @@ __device__ __forceinline__ void block_stats(const StatAcc& a, double* out) {
     for (int j = 0; j < 5; ++j) {
       double r = part[0][j];
       for (int k = 1; k < nw; ++k) r = j == 1 ? fmax(r, part[k][j]) : r + part[k][j];
+      if (ADD) r = j == 1 ? fmax(out[j], r) : out[j] + r;
       out[j] = r;
     }
   }
@@ -182,6 +230,11 @@ __global__ __launch_bounds__(256) void k_forces(Particles P, Sorted S, Grid G, c
   const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (!EXTRAS) {
     if (t < sp.n) force_one<MODE, MIXED, SHAPES, false>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, nullptr);
+    return;
+  }
+  if (SHAPES) {
+    // the narrow phase (k_narrow) collected this evaluation's statistics
+    if (t < sp.n) force_one<MODE, MIXED, true, true>(P, S, G, obs, models, sp, st, pairs, L, VS, CL, t, nullptr);
     return;
   }
   StatAcc acc;

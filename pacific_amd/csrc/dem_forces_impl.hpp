@@ -11,9 +11,9 @@ template <int MODE, bool MIXED>
 void launch_forces_shp(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
                        const HookeModel* models, const StepParams& sp, DevStatus* st, int* pairs, const NbrList& L,
                        const VerletState& V, const NbrList& CL, hipStream_t stream) {
-  const int bs = sp.stats ? 256 : kSphereBlock;
-  hipLaunchKernelGGL((k_forces<MODE, MIXED, true, true>), dim3(nblk_f(sp.n, bs)), dim3(bs), 0, stream, P, S, G, obs,
-                     models, sp, st, pairs, L, V, CL);
+  // the contacts come from the narrow phase's records: one-wave workgroups
+  hipLaunchKernelGGL((k_forces<MODE, MIXED, true, true>), dim3(nblk_f(sp.n, kSphereBlock)), dim3(kSphereBlock), 0,
+                     stream, P, S, G, obs, models, sp, st, pairs, L, V, CL);
 }
 template void launch_forces_shp<kForcesOnly, DEM_FORCES_MIXED>(const Particles&, const Sorted&, const Grid&,
     const BoxObstacle*, const HookeModel*, const StepParams&, DevStatus*, int*, const NbrList&, const VerletState&,
@@ -35,7 +35,7 @@ void launch_forces_sph(const Particles& P, const Sorted& S, const Grid& G, const
   // statistics keeps 256 threads, the granularity of its partial sums
   const int bs = sp.stats ? 256 : kSphereBlock;
   const dim3 grid(nblk_f(sp.n, bs)), block(bs);
-  if (sp.stats || sp.mem.cap)
+  if (sp.stats || sp.mem.cap || sp.fsum)
     hipLaunchKernelGGL((k_forces<MODE, MIXED, false, true>), grid, block, 0, stream, P, S, G, obs, models, sp, st,
                        pairs, L, V, CL);
   else if (!MIXED)

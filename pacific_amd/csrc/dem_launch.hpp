@@ -34,6 +34,11 @@ hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, co
 hipError_t launch_stats_fold(const double* part, int n, double* tot, double time, hipStream_t stream);
 // neighbour candidates of every sorted particle into L
 hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream);
+// narrow phase of a system with non-spherical bodies: the contacts of every
+// sorted particle (candidates of L, or its 27 cells when L.count is null)
+// into sp.cf, and the contact statistics when sp.stats is set
+hipError_t launch_narrow(const Particles& P, const Sorted& S, const Grid& G, const BoxObstacle* obs,
+                         const StepParams& sp, DevStatus* st, const NbrList& L, hipStream_t stream);
 // force sweep over the candidates of L (L.count == nullptr: 27-cell sweep)
 hipError_t launch_forces(int mode, bool mixed, const Particles& P, const Sorted& S, const Grid& G,
                          const BoxObstacle* obs, const HookeModel* models, const StepParams& sp, DevStatus* st,

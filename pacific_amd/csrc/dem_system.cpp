@@ -143,6 +143,15 @@ int grains_gpu_system::allocate() {
     mem.cap = kMemoryCap;
     mem.cur = 0;
   }
+  if (P.shape) {
+    HIPCHECK(this, track(this, &cf.count, n));
+    HIPCHECK(this, track(this, &cf.ent, (size_t)kFeatCap * n));
+    HIPCHECK(this, track(this, &cf.pid, (size_t)kFeatCap * n));
+    for (int j = 0; j < 9; ++j) HIPCHECK(this, track(this, &cf.f[j], (size_t)kFeatCap * n));
+    HIPCHECK(this, hipMemsetAsync(cf.count, 0, sizeof(int) * n, stream));
+    cf.cap = kFeatCap;
+    cf.n = (int)n;
+  }
   HIPCHECK(this, track(this, &d_stat_part, 5 * ((n + 255) / 256) + 5));
   HIPCHECK(this, track(this, &d_stat_tot, 8));
   HIPCHECK(this, hipMemsetAsync(d_stat_tot, 0, 8 * sizeof(double), stream));
@@ -314,6 +323,8 @@ StepParams grains_gpu_system::params() const {
   // the adjoint's replays are not counted again
   sp.stats = stats_on && !recording ? d_stat_part : nullptr;
   sp.mem = mem;
+  sp.cf = cf;
+  sp.fsum = fsum_on ? fsum_buf : nullptr;
   return sp;
 }
 
@@ -374,6 +385,13 @@ int grains_gpu_system::forces(int mode, int first_step, bool commit) {
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
     HIPCHECK(this, launch_detect(S, grid, nparticles, L, stream));
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
+  }
+  if (P.shape) {
+    // non-spherical bodies: the narrow phase records the contacts the force
+    // sweep reads
+    if (prof && (e = mark(prof->narrow_ev, prof->nnarrow))) return e;
+    HIPCHECK(this, launch_narrow(P, S, grid, d_obs, sp, d_status, L, stream));
+    if (prof && (e = mark(prof->narrow_ev, prof->nnarrow))) return e;
   }
   const dem::NbrList CL = contact_list();
   if (CL.count) {
@@ -533,6 +551,11 @@ int grains_gpu_system::check_status() {
       snprintf(buf, sizeof(buf), "particle %d has %d contacts with memory, more than the %d kept per particle", st.id0,
                st.id1, kMemoryCap);
       return fail(this, GRAINS_DEVICE_ERROR, buf);
+    }
+    if (st.code == kContactOverflow) {
+      snprintf(buf, sizeof(buf), "particle %d has %d contacts, more than the %d recorded per particle", st.id0,
+               st.id1, kFeatCap);
+      return fail(this, GRAINS_SIMULATION_ERROR, buf);
     }
     snprintf(buf, sizeof(buf), "%s raised on the device (bodies %d and %d)", names[st.code & 3], st.id0, st.id1);
     return fail(this, st.code, buf);
@@ -913,10 +936,12 @@ int grains_gpu_profile_steps(grains_gpu_system* s, int nsteps, double* out) {
   pr.force_ev.resize(2 * (size_t)nsteps + 2);
   pr.bin_ev.resize(2 * (size_t)nsteps + 2);
   pr.detect_ev.resize(2 * (size_t)nsteps + 2);
+  pr.narrow_ev.resize(2 * (size_t)nsteps + 2);
   int e = GRAINS_OK;
   for (auto& ev : pr.force_ev) HIPCHECK(s, hipEventCreate(&ev));
   for (auto& ev : pr.bin_ev) HIPCHECK(s, hipEventCreate(&ev));
   for (auto& ev : pr.detect_ev) HIPCHECK(s, hipEventCreate(&ev));
+  for (auto& ev : pr.narrow_ev) HIPCHECK(s, hipEventCreate(&ev));
   HIPCHECK(s, hipEventRecord(s->ev0, s->stream));
   s->prof = &pr;
   e = s->enqueue_steps(nsteps);
@@ -947,10 +972,18 @@ int grains_gpu_profile_steps(grains_gpu_system* s, int nsteps, double* out) {
     }
     out[5] = dsum;
     out[6] = (double)(pr.ndetect / 2);
+    double nsum = 0.;
+    for (size_t i = 0; i + 1 < pr.nnarrow; i += 2) {
+      HIPCHECK(s, hipEventElapsedTime(&t, pr.narrow_ev[i], pr.narrow_ev[i + 1]));
+      nsum += t;
+    }
+    out[7] = nsum;
+    out[8] = (double)(pr.nnarrow / 2);
   }
   for (auto& ev : pr.force_ev) (void)hipEventDestroy(ev);
   for (auto& ev : pr.bin_ev) (void)hipEventDestroy(ev);
   for (auto& ev : pr.detect_ev) (void)hipEventDestroy(ev);
+  for (auto& ev : pr.narrow_ev) (void)hipEventDestroy(ev);
   if (e) return e;
   return s->check_status();
 }
@@ -978,6 +1011,25 @@ int grains_gpu_compute_forces(grains_gpu_system* s) {
   int e = s->forces(kModeForcesOnly, 0, false);
   if (e) return e;
   return s->check_status();
+}
+
+int grains_gpu_force_sums(grains_gpu_system* s, double* out) {
+  if (!s || !s->initialized || !out) return GRAINS_ARGUMENT_ERROR;
+  HIPCHECK(s, hipSetDevice(s->device));
+  const int n = s->nparticles;
+  if (!s->d_fsum) HIPCHECK(s, track(s, &s->d_fsum, (size_t)n));
+  s->fsum_on = true;
+  s->fsum_buf = s->d_fsum;
+  int e = s->forces(kModeForcesOnly, 0, false);
+  s->fsum_on = false;
+  if (e) return e;
+  if ((e = s->check_status())) return e;
+  std::vector<int> ids((size_t)n);
+  std::vector<double> buf((size_t)n);
+  HIPCHECK(s, hipMemcpy(ids.data(), s->P.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHECK(s, hipMemcpy(buf.data(), s->d_fsum, sizeof(double) * n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) out[ids[i] - 1] = buf[i];
+  return GRAINS_OK;
 }
 
 int grains_gpu_get_state(grains_gpu_system* s, double* x, double* v, double* w, double* wb, double* q, double* F,

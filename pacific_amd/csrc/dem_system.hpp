@@ -96,6 +96,16 @@ struct grains_gpu_system {
     return verlet ? dem::NbrList{d_ct_count, d_ct, kContactCap, nparticles} : dem::NbrList{};
   }
 
+  // contacts found by the narrow phase of a system with non-spherical
+  // bodies, read by its force sweep (allocated when the system has any)
+  static constexpr int kFeatCap = 32;
+  dem::ContactFeat cf{};
+  // per-particle sums of the contact-force magnitudes of one evaluation
+  // (grains_gpu_force_sums; allocated on first use)
+  double* d_fsum = nullptr;
+  bool fsum_on = false;
+  double* fsum_buf = nullptr;  // buffer of the evaluation when fsum_on (d_fsum or a caller's scratch)
+
   // contact statistics (AppCollision): per-block partials of the last force
   // launch and running totals; force_time = time of the next force launch
   // HookeMemory contact memory (allocated when a model has memory)
@@ -134,8 +144,8 @@ struct grains_gpu_system {
   int forces(int mode, int first_step, bool commit = true);
   // optional per-launch HIP events (force sweeps and binning sequences)
   struct Profile {
-    std::vector<hipEvent_t> force_ev, bin_ev, detect_ev;
-    size_t nforce = 0, nbin = 0, ndetect = 0;
+    std::vector<hipEvent_t> force_ev, bin_ev, detect_ev, narrow_ev;
+    size_t nforce = 0, nbin = 0, ndetect = 0, nnarrow = 0;
   };
   Profile* prof = nullptr;
   int mark(std::vector<hipEvent_t>& evs, size_t& n);

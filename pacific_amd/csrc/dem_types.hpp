@@ -164,7 +164,28 @@ struct DevStatus {
   int n_rebuild;     // Verlet list builds
 };
 
-enum ErrCode { kOk = 0, kContactError = 1, kMotionError = 2, kSimulationError = 3, kMemoryOverflow = 4 };
+enum ErrCode {
+  kOk = 0,
+  kContactError = 1,
+  kMotionError = 2,
+  kSimulationError = 3,
+  kMemoryOverflow = 4,
+  kContactOverflow = 5
+};
+
+// Contacts of one step of a system with non-spherical bodies, as the narrow
+// phase (k_narrow) found them, for the force sweep: record j of sorted slot
+// k at [j * n + k] holds the partner (storage index >= 0, or -1 - obstacle
+// index), the partner's id, the partner's centre at its periodic image
+// (f[0..2]), the contact point (f[3..5]) and the overlap vector (f[6..8]) of
+// RigidBodyWithCrust::ClosestPoint, in the order the narrow phase met them.
+struct ContactFeat {
+  int* count;    // [n] contacts of each sorted slot (may exceed cap: error)
+  int* ent;      // [cap * n]
+  int* pid;      // [cap * n]
+  double* f[9];  // [cap * n] each
+  int cap, n;
+};
 enum Scheme { kLeapFrog = 0, kFirstOrderExplicit = 1, kSecondOrderExplicit = 2, kAdamsBashforth2 = 3 };
 
 struct StepParams {
@@ -180,6 +201,8 @@ struct StepParams {
   int first_step;     // Adams-Bashforth first step (Euler start)
   double* stats;      // contact statistics: 5 partial sums per force block (nullptr: not collected)
   ContactMemory mem;  // HookeMemory tables (mem.cap == 0: none)
+  ContactFeat cf;     // contacts found by the narrow phase (systems with non-spherical bodies)
+  double* fsum;       // per storage index: sum of the magnitudes of its contact forces (nullptr: not kept)
 };
 
 }  // namespace dem

@@ -49,7 +49,7 @@ EXPORTS = [
     "grains_gpu_neighbor_stats", "grains_gpu_set_contact_stats", "grains_gpu_contact_stats",
     "grains_gpu_set_memory_contact_model", "grains_gpu_set_particle_class", "grains_gpu_velocity_max_mean",
     "grains_gpu_coordination_numbers", "grains_gpu_write_rawdata", "grains_gpu_set_outputs", "grains_gpu_run",
-    "grains_gpu_graph_stats",
+    "grains_gpu_graph_stats", "grains_gpu_force_sums",
 ]
 
 
@@ -76,6 +76,7 @@ def lib():
         L.grains_gpu_profile_steps.argtypes = [vp, ctypes.c_int, dp]
         L.grains_gpu_synchronize.argtypes = [vp]
         L.grains_gpu_compute_forces.argtypes = [vp]
+        L.grains_gpu_force_sums.argtypes = [vp, dp]
         L.grains_gpu_get_state.argtypes = [vp] + [dp] * 7
         L.grains_gpu_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
         L.grains_gpu_get_masses.argtypes = [vp, dp, dp, dp]
@@ -370,11 +371,13 @@ class GrainsGPU:
 
     def profile_steps(self, n):
         """Runs n steps with HIP events around each kernel group; returns
-        dict(total_ms, force_ms, n_force, bin_ms, n_bin, detect_ms, n_detect)."""
-        out = np.zeros(7)
+        dict(total_ms, force_ms, n_force, bin_ms, n_bin, detect_ms, n_detect,
+        narrow_ms, n_narrow) -- narrow: the narrow phase of non-spherical bodies."""
+        out = np.zeros(9)
         self._check(lib().grains_gpu_profile_steps(self.h, int(n), _d(out)))
         return {"total_ms": out[0], "force_ms": out[1], "n_force": int(out[2]), "bin_ms": out[3],
-                "n_bin": int(out[4]), "detect_ms": out[5], "n_detect": int(out[6])}
+                "n_bin": int(out[4]), "detect_ms": out[5], "n_detect": int(out[6]), "narrow_ms": out[7],
+                "n_narrow": int(out[8])}
 
     def synchronize(self):
         self._check(lib().grains_gpu_synchronize(self.h))
@@ -388,6 +391,13 @@ class GrainsGPU:
 
     def compute_forces(self):
         self._check(lib().grains_gpu_compute_forces(self.h))
+
+    def force_sums(self):
+        """Forces at the current state and, per particle (id order), the sum of
+        the magnitudes of its contact forces (grains_gpu_force_sums)."""
+        out = np.zeros(self.n)
+        self._check(lib().grains_gpu_force_sums(self.h, _d(out)))
+        return out
 
     @property
     def time(self):

@@ -267,12 +267,15 @@ def polyhedra_box(n_side=(6, 6, 6), seed=21, spheres=False, pitch=3.8e-3, crust=
 
 
 def silo(n=200_000, seed=4, width=0.1, orifice=12e-3, pitch=3.6e-3, crust=5e-5, kn=1e4, en=0.6, muc=0.5,
-         dt=5e-6, density=2500.0, sigma=0.5):
+         dt=5e-6, density=2500.0, sigma=0.0, catch_depth=0.03):
     """Polyhedra on a lattice filling the silo from its floor up (pitch 3.6 mm,
     ~2.1 x the largest circumscribed radius), one third of each shape in
-    cyclic order, random orientations, Gaussian velocities (sigma, seed + 1)
-    so that neighbours collide within a few hundred steps.  The floor is two
-    plates leaving a 12 mm slot across the silo; four walls close it."""
+    cyclic order, random orientations, at rest (or Gaussian velocities of
+    standard deviation sigma, seed + 1).  The floor is two plates leaving a
+    12 mm slot across the silo; four walls close it and continue down to a
+    collecting floor catch_depth below the slot, so that the discharged
+    bodies stay inside the domain (a body leaving the linked-cell grid is a
+    SimulationError, LinkedCell.cpp:1505-1531)."""
     shp = _shape_set(crust)
     m = int(width / pitch)
     per_layer = m * m
@@ -290,10 +293,11 @@ def silo(n=200_000, seed=4, width=0.1, orifice=12e-3, pitch=3.6e-3, crust=5e-5, 
     half = (W - orifice) / 2
     boxes = [((half, W + 2 * t, t), (half / 2, W / 2, -t / 2), None, crust, 0),
              ((half, W + 2 * t, t), (W - half / 2, W / 2, -t / 2), None, crust, 0)]
-    boxes += _walls((0.0, 0.0, 0.0), (W, W, H), t=t, crust=crust, skip_floor=True)
+    boxes += _walls((0.0, 0.0, -catch_depth), (W, W, H), t=t, crust=crust)
     return {
         "name": f"silo_{n}",
-        "origin": (-4e-3, -4e-3, -0.03), "max": (W + 4e-3, W + 4e-3, H + 4e-3), "periodic": (0, 0, 0),
+        "origin": (-4e-3, -4e-3, -catch_depth - t - 4e-3), "max": (W + 4e-3, W + 4e-3, H + 4e-3),
+        "periodic": (0, 0, 0),
         "cellsize_factor": 1.0, "gravity": (0.0, 0.0, -9.81), "dt": dt,
         "shapes": shp, "groups": groups, "boxes": boxes,
         "models": [(0, 0, kn, en, muc, 0.0, -1.0)],
