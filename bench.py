@@ -232,13 +232,14 @@ def c1_leg(args):
     from pacific_amd import FP64, workloads
     wl = workloads.settling_box(1000)
     g = build_gpu(wl, FP64)
-    ms = g.step_timed(10000)
+    steps = args.c1_steps
+    ms = g.step_timed(steps)
     vmax, vmean = g.velocity_max_mean()
-    value = 1000 * 10000 / (ms / 1e3)
+    value = 1000 * steps / (ms / 1e3)
     out = {"c1_value": value, "c1_config": {"workload": "1000 spheres r 1 mm settling in a 20x20x60 mm box of "
                                                         "walls, FP64, Hooke kn 1e4 en 0.9 mu 0.5, dt 1e-5",
-                                            "steps": 10000, "ms": ms, "max_v_end": vmax, "mean_v_end": vmean}}
-    log(f"bench: C1 settling 1000 spheres x 10000 FP64 steps: {ms:.0f} ms -> {value:.4g} particle-steps/s")
+                                            "steps": steps, "ms": ms, "max_v_end": vmax, "mean_v_end": vmean}}
+    log(f"bench: C1 settling 1000 spheres x {steps} FP64 steps: {ms:.0f} ms -> {value:.4g} particle-steps/s")
     if not args.no_cpu:
         v, t = cpu_baseline(wl, 2000)
         out["c1_cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
@@ -332,12 +333,24 @@ def polyhedra_leg(args, prec):
     g = build_gpu(wl, prec)
     t0 = time.perf_counter()
     done = 0
+    if args.poly_state and os.path.exists(args.poly_state):
+        # the filled silo of an earlier run of the same workload (profiling
+        # runs skip the fill's dispatches)
+        st = np.load(args.poly_state)
+        g.set_basis(st["B"])
+        g.set_state(x=st["x"], v=st["v"], w=st["w"], q=st["q"], recompute=True)
+        g.set_time(float(st["t"]))
+        done = args.poly_fill
+        log(f"bench: silo state loaded from {args.poly_state}")
     while done < args.poly_fill:
         k = min(5000, args.poly_fill - done)
         g.step(k)
         g.synchronize()
         done += k
         log(f"bench: silo fill {done}/{args.poly_fill} steps ({time.perf_counter() - t0:.1f} s)")
+    if args.poly_state and not os.path.exists(args.poly_state):
+        st = g.state()
+        np.savez(args.poly_state, x=st["x"], v=st["v"], w=st["w"], q=st["q"], B=g.basis(), t=g.time)
     g.step(args.poly_warmup)
     g.synchronize()
     ms = g.step_timed(args.poly_steps)
@@ -357,8 +370,9 @@ def polyhedra_leg(args, prec):
         g64 = build_gpu(wl, FP64)
         ferr = force_error(g, g64, shapes=True)
         del g64
-        log(f"bench: polyhedra mixed vs FP64 force error {ferr['max_rel_per_particle_sum']:.2e} (per particle, "
-            f"relative to the sum of its contact-force magnitudes)")
+        log(f"bench: polyhedra mixed vs FP64 force error {ferr.get('max_rel_per_particle_sum', 0.0):.2e} "
+            f"(per particle, relative to the sum of its contact-force magnitudes; "
+            f"{ferr['particles_with_contacts']} particles in contact)")
     log(f"bench: polyhedra silo {args.poly_n}: {value:.4g} particle-steps/s, narrow phase {narrow_ms * 1e3:.1f} us, "
         f"force sweep {force_ms * 1e3:.1f} us, candidates {detect_ms * 1e3:.1f} us, contacts/particle {cbar:.3f}, "
         f"bodies in contact {touching:.3f}")
@@ -467,9 +481,14 @@ def main():
     ap.add_argument("--poly-steps", type=int, default=400)
     ap.add_argument("--poly-warmup", type=int, default=100)
     ap.add_argument("--poly-fill", type=int, default=20000, help="untimed steps of the silo fill from rest")
+    ap.add_argument("--poly-state", default="", metavar="NPZ",
+                    help="load the filled silo from this file if it exists, else save it there after the fill")
     ap.add_argument("--only-poly", action="store_true", help="run the polyhedra leg alone (profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--only-adjoint", action="store_true", help="run the adjoint leg alone (profiling)")
+    ap.add_argument("--only-batch", action="store_true", help="run the batched-ensemble leg alone (profiling)")
+    ap.add_argument("--c1-steps", type=int, default=10000)
+    ap.add_argument("--only-c1", action="store_true", help="run the 1,000-sphere settling leg alone (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -497,6 +516,14 @@ def main():
 
     if args.only_adjoint:
         print(json.dumps(adjoint_leg(args)), flush=True)
+        return
+    if args.only_batch:
+        from pacific_amd import FP64
+        print(json.dumps(batched_leg(args, FP64)), flush=True)
+        return
+    if args.only_c1:
+        args.no_cpu = True
+        print(json.dumps(c1_leg(args)), flush=True)
         return
     if args.only_poly:
         from pacific_amd import FP64, MIXED

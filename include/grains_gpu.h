@@ -254,6 +254,39 @@ int grains_gpu_velocity_max_mean(grains_gpu_system* sys, double* vmax, double* v
  * 1702-1706, reset by InitializeForce :1408-1415).  Needs contact records
  * (grains_gpu_set_record_pairs) during that evaluation. */
 int grains_gpu_coordination_numbers(grains_gpu_system* sys, int* out);
+/* The same at the current state, without records of the last step: one
+ * force evaluation with contact records, after which the torsor the time
+ * loop holds is restored (so that the next step is unchanged).  Serves the
+ * post-processing calls of a coupled solver (SaveResults_Grains). */
+int grains_gpu_coordination_at_state(grains_gpu_system* sys, int* out);
+/* The post-processing writers of the input file (<PostProcessing><Writers>
+ * <RawData>) at the current state, coordination numbers included: start != 0
+ * starts the files (AllComponents::PostProcessing_start), otherwise a frame
+ * is appended (AllComponents::PostProcessing); GrainsCoupledWithFluid::
+ * InitialPostProcessing / doPostProcessing, GrainsCoupledWithFluid.cpp:
+ * 870-893.  Nothing to do without a writer. */
+int grains_gpu_post_processing(grains_gpu_system* sys, int start);
+
+/* Reload (restart) files, __Format2024__ layout (Grains::saveReload,
+ * Grains.cpp:2543-2584; AllComponents::write, AllComponents.cpp:1299-1415;
+ * Particle::write2014 / write2014_binary, Particle.cpp:1167-1193;
+ * ContactBuilderFactory::save, ContactBuilderFactory.cpp:270-329): writes
+ * <root>A or <root>B (alternating per system, A first) with the particle
+ * types, obstacles and file table; the particle records <root>X_Particles
+ * (text, %.16e) or <root>X_Particles.bin (hybrid != 0: the binary record
+ * of write2014_binary, preceded by its byte count); <root>_MatContact.xml at
+ * the first save; one line "time <root>X" appended to <root>_RFTable.txt. */
+int grains_gpu_save_reload(grains_gpu_system* sys, const char* root, int hybrid);
+/* Reads a reload file written for the same particles (ids 1..n): time,
+ * positions, bases, velocities, quaternions, the torsor the LeapFrog
+ * records carry (no new force evaluation, as Grains.cpp:169-174 skips it
+ * on reload), Adams-Bashforth data and contact memory (Grains.cpp:784-844,
+ * AllComponents::read / read_particles, AllComponents.cpp:1024-1290). */
+int grains_gpu_load_reload(grains_gpu_system* sys, const char* file);
+/* The same from the last file listed in <RestartFile Name>_RFTable.txt of
+ * the system's input file (<Reload Type="same">; GrainsCoupledWithFluid::
+ * setReloadSame; GrainsExec::restartFileName_AorB, GrainsExec.cpp:455-469). */
+int grains_gpu_reload_same(grains_gpu_system* sys);
 
 /* One RawData frame at the current time (RawDataPostProcessingWriter::
  * one_output_Standard, RawDataPostProcessingWriter.cpp:362-600): the files
@@ -348,6 +381,14 @@ int grains_gpu_to_fluid(grains_gpu_system* sys, char** buf, int* size);
 /* m rows of (vx, vy, vz, wx, wy, wz) by id - 1
  * (GrainsCoupledWithFluid::updateParticlesVelocity, :1126-1181) */
 int grains_gpu_set_velocities(grains_gpu_system* sys, const double* vw, size_t m);
+/* m rows of the fluid's force and torque on each particle (Fx, Fy, Fz, Tx,
+ * Ty, Tz) by id - 1, added to the torsor at every force evaluation from now
+ * on, after the contact forces, the force at the centre
+ * (GrainsCoupledWithFluid::updateParticlesHydroFT, :1187-1191, ->
+ * AppPRSHydroFT::setHydroFT / ComputeForces, AppPRSHydroFT.cpp:27-63; caller
+ * FS_Grains3DPlugIn::transferHydroFTtoSolid, FS_Grains3DPlugIn.cc:181-190).
+ * m must equal the number of particles. */
+int grains_gpu_set_hydro_ft(grains_gpu_system* sys, const double* ft, size_t m);
 /* particles and obstacles sent to the fluid (numberOfRBToFluid, :896-905) */
 int grains_gpu_count_rigid_bodies(grains_gpu_system* sys, size_t* nparticles, size_t* nobstacles);
 

@@ -15,10 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgrains_gpu.so")
 CLI = os.path.join(HERE, "grains_gpu")
+EXAMPLE = os.path.join(HERE, "fs_plugin_example")
 
 SOURCES = ["dem_kernels.hip", "dem_forces_sph64.hip", "dem_forces_sph32.hip", "dem_forces_shp64.hip",
            "dem_forces_shp32.hip", "dem_narrow.hip", "dem_batch.hip", "dem_adjoint.hip", "dem_system.cpp", "dem_shapes.cpp",
-           "grains_xml.cpp", "grains_basilisk.cpp", "grains_output.cpp"]
+           "grains_xml.cpp", "grains_basilisk.cpp", "grains_output.cpp", "grains_reload.cpp"]
 ARCH = os.environ.get("PACIFIC_AMD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -67,6 +68,12 @@ def build(verbose=False):
         out += _run([hipcc()] + HIPCC_FLAGS + ["-o", tmpc, cli_src, "-L" + HERE, "-lgrains_gpu",
                                                "-Wl,-rpath,$ORIGIN"])
         os.replace(tmpc, CLI)
+    ex_src = os.path.join(CSRC, "fs_plugin_example.cpp")
+    if os.path.exists(ex_src):
+        tmpe = EXAMPLE + ".tmp"
+        out += _run([hipcc()] + HIPCC_FLAGS + ["-o", tmpe, ex_src, "-L" + HERE, "-lgrains_gpu",
+                                               "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmpe, EXAMPLE)
     if verbose and out.strip():
         print(out)
     return LIB

@@ -94,6 +94,7 @@ __global__ __launch_bounds__(512) void k_batch(const BatchSys* __restrict__ sys,
   sp.mem = ContactMemory{};
   sp.cf = ContactFeat{};
   sp.fsum = nullptr;
+  sp.hyd = nullptr;
   const BoxObstacle* bobs = obs + B.o0;
   const HookeModel* bmod = models + B.m0;
   const int nrounds = mode == 0 ? 1 : nsteps;

@@ -300,15 +300,20 @@ DEM_HD bool j_degenerate(const Simplex& S, unsigned all_bits, d3 w) {
 
 // One iteration of closest_points with the new point in slot LAST: 1 to go
 // on, 0 when the loop ends.
+// sep >= 0: 2 as soon as the support point w proves the distance larger
+// than sep (the lower bound v.w / |v| of the distance exceeds it, with a
+// margin above the rounding of v.w), so that a pair that cannot be in
+// contact stops at once; the distance itself is then not computed.
 template <int LAST>
 DEM_HD int gjk_iter(Simplex& S, const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w,
-                    unsigned& bits, unsigned& all_bits, d3& v, double& dist) {
+                    unsigned& bits, unsigned& all_bits, d3& v, double& dist, double sep) {
   constexpr unsigned last_bit = 1u << LAST;
   const double relError = kEps, absError = 1.e-4 * relError;
   d3 d = v;
   S.p[LAST] = support(A, tf_vec_basis(-d, a2w));
   S.q[LAST] = support(B, tf_vec_basis(d, b2w));
   d3 w = tf_apply(a2w, S.p[LAST]) - tf_apply(b2w, S.q[LAST]);
+  if (sep >= 0. && dot(v, w) > sep * dist + 1.e-12 * dist * (fabs(w.x) + fabs(w.y) + fabs(w.z))) return 2;
   double mu = dist - dot(v, w) / dist;
   if (mu < dist * relError || mu < absError) return 0;
   if (j_degenerate(S, all_bits, w)) return 0;
@@ -321,8 +326,10 @@ DEM_HD int gjk_iter(Simplex& S, const ConvexGeo& A, const Tf& a2w, const ConvexG
 
 // closest_points with relative tolerance 1e-10 (Convex.cpp:536-627);
 // witness points in the local frames; returns the distance
+// (sep >= 0: *separated set when the distance is proven larger than sep;
+// the witness points are then not computed)
 DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, d3& pa, d3& pb,
-                          int& iters) {
+                          int& iters, double sep = -1., bool* separated = nullptr) {
   Simplex S;
   for (int i = 0; i < 16; ++i)
     for (int j = 0; j < 4; ++j) S.det[i][j] = 0.;
@@ -336,10 +343,15 @@ DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B,
     ++it;
     // the first free slot
     int go;
-    if (!(bits & 1u)) go = gjk_iter<0>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
-    else if (!(bits & 2u)) go = gjk_iter<1>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
-    else if (!(bits & 4u)) go = gjk_iter<2>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
-    else go = gjk_iter<3>(S, A, a2w, B, b2w, bits, all_bits, v, dist);
+    if (!(bits & 1u)) go = gjk_iter<0>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
+    else if (!(bits & 2u)) go = gjk_iter<1>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
+    else if (!(bits & 4u)) go = gjk_iter<2>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
+    else go = gjk_iter<3>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
+    if (go == 2) {
+      if (separated) *separated = true;
+      iters = it;
+      return dist;
+    }
     if (!go) break;
   }
   j_points(S, bits, pa, pb);
@@ -408,8 +420,11 @@ DEM_HD bool recover_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, d
 // bodies intersect (attempt 0: go on with attempt 1; attempt 1:
 // ContactError).  A caller that evaluates every pair through one call of
 // this function holds a single copy of the GJK state in registers.
+// early: attempt 0 stops as soon as the pair is proven farther apart than
+// the crusts (no contact; the iteration count is then not the full run's,
+// so a caller collecting statistics passes false).
 DEM_HD int gjk_attempt(const ConvexGeo& A, const ConvexGeo& B, int attempt, d3& P, d3& ov, double& d,
-                       int& iters) {
+                       int& iters, bool early = false) {
   d3 sa = A.scaling, sb = B.scaling;
   if (attempt == 0) {
     d3 gcagcb = A.o - B.o;
@@ -428,7 +443,9 @@ DEM_HD int gjk_attempt(const ConvexGeo& A, const ConvexGeo& B, int attempt, d3& 
   const double crusts = A.crust + B.crust;
   Tf a2w = tf_scaled(A, sa), b2w = tf_scaled(B, sb);
   d3 pa, pb;
-  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, iters);
+  bool separated = false;
+  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, iters, early && attempt == 0 ? crusts : -1., &separated);
+  if (separated) return 0;
   if (distance < kEps) return -1;
   pa = tf_apply(a2w, pa);
   pb = tf_apply(b2w, pb);

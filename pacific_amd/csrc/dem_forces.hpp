@@ -147,6 +147,13 @@ __device__ __forceinline__ void force_one(const Particles& P, const Sorted& S, c
     sweep_particle<MIXED, false, EXTRAS>(P, S, G, obs, sp.nobs, models, sp, st, pairs, 0, 0, sp.n, cv, k, me, F, T,
                                          nullptr, cx, &L, &VS, s, &CL);
   }
+  // forces of the fluid (AppPRSHydroFT::ComputeForces, AppPRSHydroFT.cpp:
+  // 27-44): an application after the contacts, the force at the centre
+  if (sp.hyd) {
+    const size_t n = (size_t)sp.n;
+    F += d3{sp.hyd[s], sp.hyd[n + s], sp.hyd[2 * n + s]};
+    T += d3{sp.hyd[3 * n + s], sp.hyd[4 * n + s], sp.hyd[5 * n + s]};
+  }
   // inside a fused LeapFrog step the torsor is consumed here; it is stored by
   // the force-only and kick launches that end a batch
   if (!(MODE == kFused && sp.scheme == kLeapFrog)) {

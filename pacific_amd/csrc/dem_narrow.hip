@@ -155,7 +155,7 @@ __device__ __forceinline__ void narrow_one(const Particles& P, const Sorted& S, 
           continue;
         }
       } else {
-        res = gjk_attempt(mg, og, att, Pc, ov, d, nit);
+        res = gjk_attempt(mg, og, att, Pc, ov, d, nit, acc == nullptr);
         if (res < 0 && att == 0) {
           attempt = 1;
           continue;

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -325,6 +326,7 @@ StepParams grains_gpu_system::params() const {
   sp.mem = mem;
   sp.cf = cf;
   sp.fsum = fsum_on ? fsum_buf : nullptr;
+  sp.hyd = d_hyd;
   return sp;
 }
 
@@ -473,8 +475,14 @@ int grains_gpu_system::enqueue_steps_impl(int nsteps) {
 // Graph replay serves the plain time loop: no per-step host work (profile
 // events, contact records, statistics with their time stamps, adjoint
 // recording) may be part of it.
+// PACIFIC_AMD_NO_GRAPHS=1 in the environment launches every step directly
+// (profiling runs: per-kernel tracers see the same kernels either way).
 bool grains_gpu_system::graph_steps_allowed() const {
-  return !prof && !recording && !record_pairs && !stats_on && cfg.scheme == GRAINS_LEAPFROG;
+  static const bool disabled = [] {
+    const char* v = std::getenv("PACIFIC_AMD_NO_GRAPHS");
+    return v && v[0] == '1';
+  }();
+  return !disabled && !prof && !recording && !record_pairs && !stats_on && cfg.scheme == GRAINS_LEAPFROG;
 }
 
 void grains_gpu_system::drop_graphs() {
@@ -621,6 +629,16 @@ int grains_gpu_create(const grains_gpu_config* cfg, grains_gpu_system** out) {
 void grains_gpu_destroy(grains_gpu_system* sys) { delete sys; }
 
 static int set_model(grains_gpu_system* s, int matA, int matB, const HookeModel& m) {
+  if (s->initialized) {
+    // initialize() copied the default into every pair left undefined and
+    // sized the contact memory from the models then set: a default or a
+    // change of model kind now would be silently ignored
+    if (matA < 0 || matB < 0)
+      return fail(s, GRAINS_ARGUMENT_ERROR, "the default contact model is set before initialisation");
+    if (s->models[matA * kMaxMaterials + matB].memory != m.memory)
+      return fail(s, GRAINS_ARGUMENT_ERROR,
+                  "a contact model with memory replaces or is replaced by one without before initialisation");
+  }
   if (matA < 0 || matB < 0) {
     s->default_model = m;
     s->has_default = true;
@@ -1011,6 +1029,31 @@ int grains_gpu_compute_forces(grains_gpu_system* s) {
   int e = s->forces(kModeForcesOnly, 0, false);
   if (e) return e;
   return s->check_status();
+}
+
+// GrainsCoupledWithFluid::updateParticlesHydroFT (GrainsCoupledWithFluid.cpp:
+// 1187-1191) -> AppPRSHydroFT::setHydroFT: m rows of (Fx, Fy, Fz, Tx, Ty, Tz)
+// by particle id - 1, applied at every force evaluation from now on.
+int grains_gpu_set_hydro_ft(grains_gpu_system* s, const double* ft, size_t m) {
+  if (!s || !s->initialized || (!ft && m)) return GRAINS_ARGUMENT_ERROR;
+  const int n = s->nparticles;
+  if (m != (size_t)n)
+    return fail(s, GRAINS_ARGUMENT_ERROR,
+                "hydro force rows " + std::to_string(m) + " != particles " + std::to_string(n));
+  HIPCHECK(s, hipSetDevice(s->device));
+  if (n == 0) return GRAINS_OK;
+  const bool fresh = !s->d_hyd;
+  if (fresh) HIPCHECK(s, track(s, &s->d_hyd, 6 * (size_t)n));
+  std::vector<int> ids((size_t)n);
+  HIPCHECK(s, hipMemcpy(ids.data(), s->P.id, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<double> buf(6 * (size_t)n);
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < 6; ++c) buf[(size_t)c * n + i] = ft[6 * (size_t)(ids[i] - 1) + c];
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  HIPCHECK(s, hipMemcpy(s->d_hyd, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+  // step graphs captured before hold StepParams without the fluid forces
+  if (fresh) s->drop_graphs();
+  return GRAINS_OK;
 }
 
 int grains_gpu_force_sums(grains_gpu_system* s, double* out) {

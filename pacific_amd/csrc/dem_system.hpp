@@ -60,6 +60,11 @@ struct grains_gpu_system {
   double coupled_min_dt = 0., coupled_max_dt = 0.;
   size_t coupled_ndt = 0;
   std::vector<int> obs_to_fluid;
+  // names the reload files carry (set by the XML front-end; defaults
+  // otherwise): materials by index, polyhedron files by shape class,
+  // obstacles by index; saves made so far (A/B alternation)
+  std::vector<std::string> mat_names, shape_files, obs_names;
+  int reload_count = 0;
   std::vector<dem::BoxObstacle> obstacles;
   dem::HookeModel models[dem::kMaxMaterials * dem::kMaxMaterials]{};
   dem::HookeModel default_model{};
@@ -102,6 +107,9 @@ struct grains_gpu_system {
   dem::ContactFeat cf{};
   // per-particle sums of the contact-force magnitudes of one evaluation
   // (grains_gpu_force_sums; allocated on first use)
+  // forces and torques of the fluid on each particle (AppPRSHydroFT), 6 x n
+  // by storage index; allocated by the first grains_gpu_set_hydro_ft
+  double* d_hyd = nullptr;
   double* d_fsum = nullptr;
   bool fsum_on = false;
   double* fsum_buf = nullptr;  // buffer of the evaluation when fsum_on (d_fsum or a caller's scratch)

@@ -203,6 +203,7 @@ struct StepParams {
   ContactMemory mem;  // HookeMemory tables (mem.cap == 0: none)
   ContactFeat cf;     // contacts found by the narrow phase (systems with non-spherical bodies)
   double* fsum;       // per storage index: sum of the magnitudes of its contact forces (nullptr: not kept)
+  const double* hyd;  // fluid force and torque per storage index, component c at hyd[c * n + s] (nullptr: none)
 };
 
 }  // namespace dem

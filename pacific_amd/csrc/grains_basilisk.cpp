@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -255,18 +256,21 @@ int grains_gpu_to_fluid(grains_gpu_system* s, char** buf, int* size) {
   std::string text;
   int e = to_fluid_text(s, text);
   if (e) return e;
-  *size = int(text.size()) + 1;
-  char* out = (char*)std::malloc((size_t)*size);
+  // every token is followed by one space: at most text.size() + 1 bytes
+  // before the terminating NUL
+  const size_t cap = text.size() + 2;
+  char* out = (char*)std::malloc(cap);
   if (!out) return GRAINS_DEVICE_ERROR;
   std::istringstream iss(text);
   std::string tok;
   size_t pos = 0;
-  while (iss >> tok) {
+  while (iss >> tok && pos + tok.size() + 1 < cap) {
     std::memcpy(out + pos, tok.data(), tok.size());
     pos += tok.size();
     out[pos++] = ' ';
   }
   out[pos] = '\0';
+  *size = int(pos) + 1;
   *buf = out;
   return GRAINS_OK;
 }
@@ -302,13 +306,14 @@ int grains_gpu_count_rigid_bodies(grains_gpu_system* s, size_t* nparticles, size
 
 void Init_Grains(char const* inputfile, double fluid_rho, const bool b_restart, const bool b_fluidcorrectedacc) {
   if (g_sys) die("Init_Grains called twice");
-  if (b_restart) die("restarting from Grains3D reload files is not supported by this library");
   grains_gpu_system* s = nullptr;
   int e = grains_gpu_create_coupled(inputfile, fluid_rho, b_fluidcorrectedacc ? 1 : 0, &s);
   if (e) {
     std::string msg = s ? s->last_error : std::string("cannot build the system");
     die(msg.c_str());
   }
+  // GrainsCoupledWithFluid::setReloadSame: the last reload file of the table
+  if (b_restart && grains_gpu_reload_same(s)) die(grains_gpu_last_error(s));
   g_sys = s;
   std::printf("Construction of Grains completed\n");
 }
@@ -330,16 +335,23 @@ char* GrainsToBasilisk(int* pstrsize) {
 
 void SetInitialTime(double tinit) { grains_gpu_set_time(sys(), tinit); }
 
+// InterfaceGrains3DBasilisk.cpp:83-92: the first call starts the
+// post-processing writers (InitialPostProcessing -> PostProcessing_start,
+// files truncated), the next ones append a frame (doPostProcessing).  The
+// writer this library has is RawData (<PostProcessing><Writers><RawData>
+// of the input file): one frame of the current state, coordination
+// numbers included.
 void SaveResults_Grains(void) {
-  static bool told = false;
-  if (!told) std::printf("Grains: post-processing writers are not part of this library; use grains_gpu_get_state\n");
-  told = true;
+  static unsigned int ppcounter = 0;
+  grains_gpu_system* s = sys();
+  if (grains_gpu_post_processing(s, ppcounter == 0 ? 1 : 0)) die(grains_gpu_last_error(s));
+  ++ppcounter;
 }
 
-void checkParaviewPostProcessing_Grains(char* solid_resDir) {
-  (void)solid_resDir;
-  SaveResults_Grains();
-}
+// GrainsCoupledWithFluid::checkParaviewPostProcessing prepares the Paraview
+// writer's directory; this library has no Paraview writer, so there is
+// nothing to check.
+void checkParaviewPostProcessing_Grains(char* solid_resDir) { (void)solid_resDir; }
 
 void UpdateVelocityGrains(double** arrayv, const int m) {
   grains_gpu_system* s = sys();

@@ -110,6 +110,47 @@ int grains_gpu_coordination_numbers(grains_gpu_system* s, int* out) {
   return GRAINS_OK;
 }
 
+// Coordination numbers of the current state: a force evaluation with
+// contact records, then the torsor the time loop holds is put back, so that
+// the next step is the one it would have been.
+int grains_gpu_coordination_at_state(grains_gpu_system* s, int* out) {
+  if (!s || !s->initialized || !out) return GRAINS_ARGUMENT_ERROR;
+  const size_t n = (size_t)s->nparticles;
+  std::vector<double> ft(6 * n);
+  int e = hipSetDevice(s->device) == hipSuccess ? GRAINS_OK : GRAINS_DEVICE_ERROR;
+  if (e) return fail(s, e, "hipSetDevice failed");
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(s, GRAINS_DEVICE_ERROR, "stream sync failed");
+  for (int c = 0; c < 6 && n; ++c) {
+    double* src = c < 3 ? s->P.F[c] : s->P.T[c - 3];
+    if (hipMemcpy(ft.data() + c * n, src, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(s, GRAINS_DEVICE_ERROR, "torsor copy failed");
+  }
+  const int rec0 = s->record_pairs;
+  e = grains_gpu_set_record_pairs(s, 1);
+  if (!e) e = grains_gpu_compute_forces(s);
+  if (!e) e = grains_gpu_coordination_numbers(s, out);
+  grains_gpu_set_record_pairs(s, rec0);
+  for (int c = 0; c < 6 && n; ++c) {
+    double* dst = c < 3 ? s->P.F[c] : s->P.T[c - 3];
+    if (hipMemcpy(dst, ft.data() + c * n, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(s, GRAINS_DEVICE_ERROR, "torsor copy failed");
+  }
+  return e;
+}
+
+// The writers of the input file's <PostProcessing> at the current state
+// (AllComponents::PostProcessing_start / PostProcessing, called by
+// GrainsCoupledWithFluid::InitialPostProcessing / doPostProcessing,
+// GrainsCoupledWithFluid.cpp:870-893): start != 0 starts the files.
+int grains_gpu_post_processing(grains_gpu_system* s, int start) {
+  if (!s || !s->initialized) return GRAINS_ARGUMENT_ERROR;
+  if (s->rawdata_root.empty()) return GRAINS_OK;
+  std::vector<int> coord((size_t)s->nparticles, 0);
+  int e = grains_gpu_coordination_at_state(s, coord.data());
+  if (e) return e;
+  return grains_gpu_write_rawdata(s, s->rawdata_root.c_str(), s->rawdata_binary, start ? 1 : 0, coord.data());
+}
+
 int grains_gpu_write_rawdata(grains_gpu_system* s, const char* root, int binary, int truncate, const int* coord) {
   if (!s || !s->initialized || !root || !*root) return GRAINS_ARGUMENT_ERROR;
   const size_t n = (size_t)s->nparticles;

@@ -285,6 +285,7 @@ struct Material {
 // the reference particle's angular position (inherited by inserted
 // particles, InitialAngularPosition Type="Fixed")
 struct ParticleClass {
+  std::string file;  // polyhedron file name as given in the input
   int number;
   double density, crust, radius;
   int type;  // dem::ShapeKind
@@ -380,6 +381,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
   struct BoxDef {
     double L[3], centre[3], B[9], crust;
     int mat, to_fluid;
+    std::string name;
   };
   std::vector<BoxDef> boxes;
   struct ModelDef {
@@ -400,6 +402,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
   bool rawdata_binary = false;
   double coupled_dt[2] = {0., 0.};
   size_t coupled_ndt = 0;
+  std::string reload_type, reload_file, reload_velocity;
 
   if (c.ok && root.name != "Grains3D") c.bad("root element must be <Grains3D>");
   if (c.ok && !opt.coupled && root.has("Type") && root.attr("Type") != "Standard")
@@ -449,7 +452,18 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
           else if (v != "OFF") c.bad(std::string(opt_name) + " should be ON or OFF");
         }
       }
-    if (cons->child("Reload")) c.bad("<Reload> is not supported by the XML front-end; use grains_gpu_set_state");
+    // <Reload Type="new" Filename="..."> or <Reload Type="same"> (the last
+    // file of <RestartFile Name>_RFTable.txt), Velocity="Reset" optional
+    // (Grains.cpp:781-844): the particles the input file inserts take the
+    // state of the reload file (same ids and classes); obstacles and
+    // contact models are those of the input file
+    if (const XNode* rl = cons->child("Reload")) {
+      reload_type = rl->attr("Type");
+      if (reload_type != "new" && reload_type != "same") c.bad("<Reload Type> must be \"new\" or \"same\"");
+      reload_file = rl->attr("Filename");
+      reload_velocity = rl->attr("Velocity");
+      if (c.ok && reload_type == "new" && reload_file.empty()) c.bad("<Reload Type=\"new\"> needs a Filename");
+    }
     if (cons->child("CompositeParticles")) c.bad("<CompositeParticles> are not supported");
     if (const XNode* ps = cons->child("Particles")) {
       for (const XNode* p : ps->children("Particle")) {
@@ -472,6 +486,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
         } else if (const XNode* ph = cv->child("Polyhedron")) {
           sc.type = dem::kShapePoly;
           std::string name = ph->attr("Name"), err;
+          sc.file = name;
           std::ifstream probe(name);
           if (!probe) name = dir + name;
           if (!read_polyhedron(name, sc, err)) { c.bad(err); break; }
@@ -519,6 +534,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
         basis_of(n, b.B, c);
         const XNode* m = n->child("Material");
         b.mat = mats.define(m ? trim(m->text) : std::string(""), c);
+        b.name = n->attr("name");
         // <Status ToFluid="1"/> (SimpleObstacle.cpp:54-57)
         const XNode* stn = n->child("Status");
         b.to_fluid = stn ? std::atoi(stn->attr("ToFluid").c_str()) : 0;
@@ -780,6 +796,35 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
     e = grains_gpu_add_box(s, b.L, b.centre, b.B, b.crust, b.mat);
     if (e) return e;
     s->obs_to_fluid.back() = b.to_fluid;
+    s->obs_names.push_back(b.name);
   }
-  return grains_gpu_initialize(s);
+  s->mat_names.assign(dem::kMaxMaterials, std::string());
+  for (auto& kv : mats.index) s->mat_names[(size_t)kv.second] = kv.first;
+  for (size_t i = 0; i < classes.size(); ++i)
+    if (shape_of[i] >= 0) {
+      if (s->shape_files.size() <= (size_t)shape_of[i]) s->shape_files.resize((size_t)shape_of[i] + 1);
+      s->shape_files[(size_t)shape_of[i]] = classes[i].file;
+    }
+  if ((e = grains_gpu_initialize(s))) return e;
+  if (reload_type.empty()) return GRAINS_OK;
+  std::string file = reload_file;
+  if (reload_type == "same") {
+    // GrainsExec::restartFileName_AorB: the last entry of the table
+    std::ifstream rft(save_root + "_RFTable.txt");
+    std::string tt, rf;
+    file = save_root;
+    while (rft >> tt >> rf) file = rf;
+  }
+  std::ifstream probe(file);
+  if (!probe && !file.empty() && file[0] != '/') file = dir + file;
+  if ((e = grains_gpu_load_reload(s, file.c_str()))) return e;
+  if (reload_velocity == "Reset") {
+    // AllComponents::resetKinematics -> ParticleKinematics::reset
+    const size_t n = (size_t)s->nparticles;
+    std::vector<double> z(3 * n, 0.), q(4 * n, 0.);
+    for (size_t i = 0; i < n; ++i) q[4 * i + 3] = 1.;
+    if ((e = grains_gpu_set_state(s, nullptr, z.data(), z.data(), q.data(), 0))) return e;
+  }
+  // the reload's time is the start of the run (Grains.cpp:822-826)
+  return GRAINS_OK;
 }

@@ -49,7 +49,9 @@ EXPORTS = [
     "grains_gpu_neighbor_stats", "grains_gpu_set_contact_stats", "grains_gpu_contact_stats",
     "grains_gpu_set_memory_contact_model", "grains_gpu_set_particle_class", "grains_gpu_velocity_max_mean",
     "grains_gpu_coordination_numbers", "grains_gpu_write_rawdata", "grains_gpu_set_outputs", "grains_gpu_run",
-    "grains_gpu_graph_stats", "grains_gpu_force_sums",
+    "grains_gpu_graph_stats", "grains_gpu_force_sums", "grains_gpu_set_hydro_ft",
+    "grains_gpu_coordination_at_state", "grains_gpu_post_processing", "grains_gpu_save_reload",
+    "grains_gpu_load_reload", "grains_gpu_reload_same",
 ]
 
 
@@ -89,6 +91,11 @@ def lib():
         L.grains_gpu_set_particle_class.argtypes = [vp, ctypes.c_int]
         L.grains_gpu_velocity_max_mean.argtypes = [vp, dp, dp]
         L.grains_gpu_coordination_numbers.argtypes = [vp, ip]
+        L.grains_gpu_coordination_at_state.argtypes = [vp, ip]
+        L.grains_gpu_post_processing.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_save_reload.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.grains_gpu_load_reload.argtypes = [vp, ctypes.c_char_p]
+        L.grains_gpu_reload_same.argtypes = [vp]
         L.grains_gpu_write_rawdata.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ip]
         L.grains_gpu_set_outputs.argtypes = [vp, dp, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.grains_gpu_run.argtypes = [vp, ctypes.c_int]
@@ -134,6 +141,7 @@ def lib():
         L.grains_gpu_fluid_step.argtypes = [vp, ctypes.c_double, ip]
         L.grains_gpu_to_fluid.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), ip]
         L.grains_gpu_set_velocities.argtypes = [vp, dp, sz]
+        L.grains_gpu_set_hydro_ft.argtypes = [vp, dp, sz]
         L.grains_gpu_count_rigid_bodies.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         _lib = L
     return _lib
@@ -237,6 +245,21 @@ class GrainsGPU:
     def set_velocities(self, vw):
         vw = _c64(vw, (-1, 6))
         self._check(lib().grains_gpu_set_velocities(self.h, _d(vw), len(vw)))
+
+    def set_hydro_ft(self, ft):
+        """Fluid force and torque on each particle, rows (Fx, Fy, Fz, Tx, Ty, Tz)
+        by id - 1 (GrainsCoupledWithFluid::updateParticlesHydroFT)."""
+        ft = _c64(ft, (-1, 6))
+        self._check(lib().grains_gpu_set_hydro_ft(self.h, _d(ft), len(ft)))
+
+    def save_reload(self, root, hybrid=False):
+        """__Format2024__ reload files <root>A|B, <root>A|B_Particles[.bin],
+        <root>_MatContact.xml, <root>_RFTable.txt (Grains::saveReload)."""
+        self._check(lib().grains_gpu_save_reload(self.h, os.fsencode(str(root)), 1 if hybrid else 0))
+
+    def load_reload(self, path):
+        """State, torsor, contact memory and time from a reload file."""
+        self._check(lib().grains_gpu_load_reload(self.h, os.fsencode(str(path))))
 
     def count_rigid_bodies(self):
         a, b = ctypes.c_size_t(), ctypes.c_size_t()
@@ -402,6 +425,9 @@ class GrainsGPU:
     @property
     def time(self):
         return lib().grains_gpu_time(self.h)
+
+    def set_time(self, t):
+        lib().grains_gpu_set_time(self.h, float(t))
 
     def state(self):
         n = self.n

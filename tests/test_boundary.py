@@ -59,3 +59,34 @@ def test_argument_errors_are_reported():
         g.add_spheres(-1.0, 2500.0, 5e-5, 0, [[0.005, 0.005, 0.005]])
     with pytest.raises(GrainsError):
         g.contact_model(0, 0, 1e4, 1.5, 0.5)
+
+
+def test_cartesian_plugin_shim_compiles_and_links(tmp_path):
+    # include/GrainsCoupledWithFluid.hh (the C++ calls of FS_Grains3DPlugIn
+    # over the C ABI) compiles as plain C++ and links against the library
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if not cxx:
+        pytest.skip("no g++")
+    src = tmp_path / "plugin.cpp"
+    src.write_text('#include "GrainsCoupledWithFluid.hh"\n'
+                   "int main(int argc, char** argv) {\n"
+                   "  if (argc < 2) return 0;\n"
+                   "  GrainsCoupledWithFluid* g = GrainsCoupledWithFluid::create(argv[1], 1000.);\n"
+                   "  g->setFluidCorrectedAcceleration(true);\n"
+                   "  g->do_before_time_stepping();\n"
+                   "  std::vector<std::vector<double> > ft(1, std::vector<double>(6, 0.));\n"
+                   "  g->updateParticlesHydroFT(&ft);\n"
+                   "  g->Simulation(1e-4);\n"
+                   "  std::istringstream is;\n"
+                   "  g->GrainsToFluid(is);\n"
+                   "  g->doPostProcessing();\n"
+                   "  delete g;\n"
+                   "  return 0;\n"
+                   "}\n")
+    lib_dir = os.path.join(ROOT, "pacific_amd")
+    r = subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", lib_dir, "-lgrains_gpu", "-Wl,--allow-shlib-undefined", "-o", str(tmp_path / "plugin")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -210,3 +210,99 @@ def test_basilisk_entry_points_match_handles(gpu, tmp_path):
     a, b = ctypes.c_size_t(), ctypes.c_size_t()
     L.NumberOfRigidBodiesInBasilisk(ctypes.byref(a), ctypes.byref(b))
     assert (a.value, b.value) == g.count_rigid_bodies()
+
+
+# ---------------------------------------------------------------------------
+# Fluid forces on the particles (AppPRSHydroFT) and the post-processing of a
+# coupled run
+
+def test_hydro_force_and_torque_closed_form(gpu):
+    # one sphere without gravity under a constant fluid force and torque:
+    # LeapFrog with a constant load gives x = x0 + F t^2 / 2m, v = F t / m
+    # and, the sphere being isotropic, w = T t / I; a second sphere without
+    # fluid load stays where it is
+    g = GrainsGPU((0, 0, 0), (0.02, 0.02, 0.02), gravity=(0, 0, 0), dt=1e-5)
+    g.contact_model(0, 0, 1e4, 0.8, 0.4)
+    x0 = np.array([[0.01, 0.01, 0.01], [0.004, 0.004, 0.004]])
+    g.add_spheres(1e-3, 2500.0, 5e-5, 0, x0)
+    g.init()
+    F, T = np.array([2e-6, -1e-6, 3e-6]), np.array([1e-9, 0.0, -2e-9])
+    g.set_hydro_ft([list(F) + list(T), [0.0] * 6])
+    g.compute_forces()  # the torsor of the first half kick carries the fluid load
+    s0 = g.state()
+    assert np.array_equal(s0["F"][0], F) and np.array_equal(s0["T"][0], T)
+    n = 400
+    g.step(n)
+    s = g.state()
+    m, I, _ = g.masses()
+    t = n * 1e-5
+    # the displacement is 1e-4 of the position: its rounding over 400 drifts
+    # is ~1e-9 of it
+    assert np.allclose(s["x"][0] - x0[0], F / (2 * m[0]) * t * t, rtol=1e-8, atol=0)
+    assert np.allclose(s["v"][0], F / m[0] * t, rtol=1e-12, atol=0)
+    assert np.allclose(s["w"][0], T / I[0] * t, rtol=1e-10, atol=1e-14 * np.abs(T / I[0] * t).max())
+    assert np.array_equal(s["x"][1], x0[1]) and not s["v"][1].any()
+
+
+def test_post_processing_frames(gpu, tmp_path):
+    # grains_gpu_post_processing: the first call starts the RawData files,
+    # the next ones append; the frames hold the state and the coordination
+    # numbers of the state, and the time loop goes on unchanged
+    from pacific_amd import rawdata
+    root = str(tmp_path / "pp" / "grains")
+    os.makedirs(os.path.dirname(root))
+    text = open(coupled_case(tmp_path)).read().replace(
+        "  </Simulation>", f'    <PostProcessing>\n      <Writers>\n        <RawData Name="{root}"/>\n'
+                           "      </Writers>\n    </PostProcessing>\n  </Simulation>")
+    path = tmp_path / "coupled_pp.xml"
+    path.write_text(text)
+    g = GrainsGPU.coupled(str(path), fluid_rho=1000.0)
+    h = GrainsGPU.coupled(str(path), fluid_rho=1000.0)
+    L = lib()
+    assert L.grains_gpu_post_processing(g.h, 1) == 0
+    s0 = g.state()
+    for _ in range(3):
+        g.fluid_step(3e-5)
+        h.fluid_step(3e-5)
+    assert L.grains_gpu_post_processing(g.h, 0) == 0
+    s1 = g.state()
+    r = rawdata.read_rawdata(root, g.n)
+    assert r["position_x"].shape == (2, g.n)
+    assert np.allclose(r["position_z"][1], s1["x"][:, 2], rtol=1e-6, atol=0)
+    h.record_pairs(True)
+    h.compute_forces()
+    assert np.array_equal(r["coordination"][1], h.coordination_numbers())
+    # the evaluation behind the coordination numbers left the run unchanged
+    g.fluid_step(3e-5)
+    h2 = GrainsGPU.coupled(str(path), fluid_rho=1000.0)
+    for _ in range(4):
+        h2.fluid_step(3e-5)
+    assert np.array_equal(g.state()["x"], h2.state()["x"])
+    del s0
+
+
+def test_fs_plugin_example_program(gpu, tmp_path):
+    # the C++ plug-in calls (include/GrainsCoupledWithFluid.hh) in a compiled
+    # program: a fluid force on particle 1 over 5 fluid steps; its
+    # GrainsToFluid text equals the handle-based calls'
+    import subprocess
+    from pacific_amd.grains import LIB_PATH
+    exe = os.path.join(os.path.dirname(LIB_PATH), "fs_plugin_example")
+    assert os.path.exists(exe), "built by pacific_amd.build"
+    path = coupled_case(tmp_path)
+    F = (3e-5, -2e-5, 4e-5)
+    out = subprocess.run([exe, path, "1000", "5", "3e-5"] + [repr(f) for f in F], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    g = GrainsGPU.coupled(path, fluid_rho=1000.0)
+    rows = np.zeros((g.n, 6))
+    rows[0, :3] = F
+    for _ in range(5):
+        g.set_hydro_ft(rows)
+        g.fluid_step(3e-5)
+    assert out.stdout.split() == g.to_fluid().split()
+    # the fluid force moved particle 1 off its path without it
+    h = GrainsGPU.coupled(path, fluid_rho=1000.0)
+    for _ in range(5):
+        h.fluid_step(3e-5)
+    assert not np.array_equal(g.state()["v"][0], h.state()["v"][0])

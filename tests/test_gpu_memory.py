@@ -146,3 +146,19 @@ def test_xml_hooke_memory(gpu, tmp_path):
     g = GrainsGPU.from_xml(path)
     g.step(500)
     assert g.n == 300
+
+
+def test_model_changes_after_init_refused(gpu):
+    """initialize() fills undefined pairs from the default model and sizes
+    the contact memory: a default, or a change of model kind, set afterwards
+    would be silently lost, so the call fails; a plain model change of the
+    same kind takes effect."""
+    g = GrainsGPU((0, 0, 0), (0.02, 0.02, 0.02), gravity=(0, 0, 0), dt=1e-6)
+    g.contact_model(0, 0, 1e4, 0.6, 0.4)
+    g.add_spheres(1e-3, RHO, 5e-5, 0, np.array([[0.005, 0.01, 0.01], [0.007, 0.01, 0.01]]))
+    g.init()
+    with pytest.raises(GrainsError, match="default contact model"):
+        g.contact_model(-1, -1, 2e4, 0.6, 0.4)
+    with pytest.raises(GrainsError, match="with memory"):
+        g.memory_contact_model(0, 0, 1e4, 0.6, 0.4, 2857.0)
+    g.contact_model(0, 0, 2e4, 0.6, 0.4)
