@@ -182,6 +182,16 @@ int grains_gpu_compute_forces(grains_gpu_system* sys);
  * forces it received: the normaliser of per-particle force-error figures.
  * No counterpart in the reference (a diagnostic of this library). */
 int grains_gpu_force_sums(grains_gpu_system* sys, double* out);
+/* The same sums into the caller's DEVICE buffer d_out (n doubles, id order),
+ * allocating nothing: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The per-particle partial sums
+ * live in a static device array of 4,194,304 doubles (32 MiB per GPU, held
+ * until the process ends): larger systems are refused with
+ * GRAINS_ARGUMENT_ERROR.  Every call on one GPU shares that array, so calls
+ * that overlap on one GPU (several threads, several systems on different
+ * streams) race; calls in sequence are safe. */
+int grains_gpu_force_sums_noalloc(grains_gpu_system* sys, double* d_out);
 
 /* State in id order: x, v, w, wb, F, T: 3 doubles per particle; q: 4 (x, y,
  * z, w).  NULL pointers are skipped. */

@@ -18,7 +18,7 @@ CLI = os.path.join(HERE, "grains_gpu")
 EXAMPLE = os.path.join(HERE, "fs_plugin_example")
 
 SOURCES = ["dem_kernels.hip", "dem_forces_sph64.hip", "dem_forces_sph32.hip", "dem_forces_shp64.hip",
-           "dem_forces_shp32.hip", "dem_narrow.hip", "dem_batch.hip", "dem_adjoint.hip", "dem_system.cpp", "dem_shapes.cpp",
+           "dem_forces_shp32.hip", "dem_narrow.hip", "dem_noalloc.hip", "dem_batch.hip", "dem_adjoint.hip", "dem_system.cpp", "dem_shapes.cpp",
            "grains_xml.cpp", "grains_basilisk.cpp", "grains_output.cpp", "grains_reload.cpp"]
 ARCH = os.environ.get("PACIFIC_AMD_ARCH", "gfx950")
 

@@ -51,7 +51,7 @@ EXPORTS = [
     "grains_gpu_coordination_numbers", "grains_gpu_write_rawdata", "grains_gpu_set_outputs", "grains_gpu_run",
     "grains_gpu_graph_stats", "grains_gpu_force_sums", "grains_gpu_set_hydro_ft",
     "grains_gpu_coordination_at_state", "grains_gpu_post_processing", "grains_gpu_save_reload",
-    "grains_gpu_load_reload", "grains_gpu_reload_same",
+    "grains_gpu_load_reload", "grains_gpu_reload_same", "grains_gpu_force_sums_noalloc",
 ]
 
 
@@ -79,6 +79,7 @@ def lib():
         L.grains_gpu_synchronize.argtypes = [vp]
         L.grains_gpu_compute_forces.argtypes = [vp]
         L.grains_gpu_force_sums.argtypes = [vp, dp]
+        L.grains_gpu_force_sums_noalloc.argtypes = [vp, vp]
         L.grains_gpu_get_state.argtypes = [vp] + [dp] * 7
         L.grains_gpu_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
         L.grains_gpu_get_masses.argtypes = [vp, dp, dp, dp]
