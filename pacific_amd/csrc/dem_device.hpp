@@ -1052,6 +1052,7 @@ constexpr unsigned kNbrSlotMask = (1u << kNbrSlotBits) - 1u;
 __device__ __forceinline__ unsigned nbr_entry(int k2, int ix, int iy, int iz) {
   return (unsigned)k2 | ((unsigned)((ix + 1) + 3 * (iy + 1) + 9 * (iz + 1)) << kNbrSlotBits);
 }
+constexpr int kNbrNoShift = 13;  // the image code of a neighbour without periodic shift
 
 // The same contacts as for_each_contact, from the candidate list written by
 // k_detect for this step (same candidates, same order).  A particle whose

@@ -78,14 +78,24 @@ __device__ __forceinline__ void listed_contacts(const Particles& P, const Sorted
   vis.s = s;
   const ContactFeat& cf = sp.cf;
   const int c = min(cf.count[k], cf.cap);
-  for (int j = 0; j < c; ++j) {
-    const size_t e = (size_t)j * cf.n + k;
-    const int o = cf.ent[e];
-    const d3 Pc{cf.f[3][e], cf.f[4][e], cf.f[5][e]}, ov{cf.f[6][e], cf.f[7][e], cf.f[8][e]};
-    if (o >= 0) {
-      if (o < sp.n) vis.pair(-1, o, cf.pid[e], d3{cf.f[0][e], cf.f[1][e], cf.f[2][e]}, 0., Pc, ov, -1.);
-    } else if (-1 - o < sp.nobs) {
-      vis.wall(-1 - o, Pc, ov, -1.);
+  // the slots are read 16 at a time with independent loads, and only those
+  // holding a contact (not kNoFeat) are evaluated, in slot order
+  for (int j0 = 0; j0 < c; j0 += 16) {
+    unsigned live = 0u;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (j0 + u < c && cf.ent[(size_t)(j0 + u) * cf.n + k] != kNoFeat) live |= 1u << u;
+    while (live) {
+      const int j = j0 + __builtin_ctz(live);
+      live &= live - 1u;
+      const size_t e = (size_t)j * cf.n + k;
+      const int o = cf.ent[e];
+      const d3 Pc{cf.f[3][e], cf.f[4][e], cf.f[5][e]}, ov{cf.f[6][e], cf.f[7][e], cf.f[8][e]};
+      if (o >= 0) {
+        if (o < sp.n) vis.pair(-1, o, cf.pid[e], d3{cf.f[0][e], cf.f[1][e], cf.f[2][e]}, 0., Pc, ov, -1.);
+      } else if (-1 - o < sp.nobs) {
+        vis.wall(-1 - o, Pc, ov, -1.);
+      }
     }
   }
   F = vis.F;

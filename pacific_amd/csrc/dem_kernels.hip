@@ -220,9 +220,15 @@ __global__ __launch_bounds__(256) void k_canon(Particles P, Sorted S, int ncells
 // particle's position and the nine row ranges, the kernel keeps few
 // registers live, so many waves per SIMD overlap its dependent loads; the
 // force sweep then reads a short list instead of 27 cells.
-__global__ __launch_bounds__(256) void k_detect(Sorted S, Grid G, int n, NbrList L) {
+// mark (systems with non-spherical bodies): each listed slot's contact
+// record starts as kNoFeat, so that the narrow phase writes only contacts.
+// After an error raised on the device (which ends the run) the contact
+// searches return at once, so that the rest of a batch of steps costs
+// little whatever the state has become.
+__global__ __launch_bounds__(256) void k_detect(Sorted S, Grid G, int n, NbrList L, int* __restrict__ mark,
+                                                const DevStatus* st) {
   const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n || st->code) return;
   const double* pr = S.posr + 4 * (size_t)k;
   const d3 x{pr[0], pr[1], pr[2]};
   const double r = pr[3];
@@ -238,7 +244,10 @@ __global__ __launch_bounds__(256) void k_detect(Sorted S, Grid G, int n, NbrList
     double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
     double R = r + q2[3];
     if (s2 > R * R * 1.000000000001) return;
-    if (c < L.cap) L.ent[(size_t)c * n + k] = nbr_entry(k2, ix, iy, iz);
+    if (c < L.cap) {
+      L.ent[(size_t)c * n + k] = nbr_entry(k2, ix, iy, iz);
+      if (mark) mark[(size_t)c * n + k] = kNoFeat;
+    }
     ++c;
   });
   L.count[k] = c;
@@ -296,7 +305,7 @@ __global__ __launch_bounds__(256) void k_vfilter(Particles P, Sorted S, Grid G, 
   // this step's drift
   if (blockIdx.x == 0 && threadIdx.x == 0) st->rebuild = 0;
   const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || st->code) return;
   const d3 x = ld3(P.x[cx], s);
   const double r = P.rad[s];
   int c2 = 0;
@@ -401,8 +410,9 @@ __global__ __launch_bounds__(256) void k_vkey(Particles P, Sorted S, VerletState
 }
 
 __global__ __launch_bounds__(256) void k_vdetect(Sorted S, VerletState V, int n, NbrList L, const int* gate,
-                                                 int stamp) {
+                                                 int stamp, const DevStatus* st) {
   if (gate && *gate != stamp) return;  // Verlet list still valid: nothing to rebuild
+  if (st->code) return;                // an error ended the run
   const int bs = blockDim.x;
   for (int blk = xcd_tile(blockIdx.x, gridDim.x); blk * bs < n; blk += gridDim.x) {
     const int k = blk * bs + threadIdx.x;
@@ -440,7 +450,7 @@ hipError_t launch_verlet_build(const Particles& P, const Sorted& S, const Verlet
   // one wave per workgroup (the candidate loops differ in length from
   // particle to particle)
   const int nb1 = stamp ? std::min(nblk(n, 64), 4 * kGatedBlocks) : nblk(n, 64);
-  hipLaunchKernelGGL(k_vdetect, dim3(nb1), dim3(64), 0, stream, S, V, n, L, gate, stamp);
+  hipLaunchKernelGGL(k_vdetect, dim3(nb1), dim3(64), 0, stream, S, V, n, L, gate, stamp, st);
   return hipGetLastError();
 }
 
@@ -463,9 +473,10 @@ hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, co
   return hipGetLastError();
 }
 
-hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream) {
+hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, const DevStatus* st,
+                         hipStream_t stream, int* mark) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_detect, dim3(nblk(n, 256)), dim3(256), 0, stream, S, G, n, L);
+  hipLaunchKernelGGL(k_detect, dim3(nblk(n, 256)), dim3(256), 0, stream, S, G, n, L, mark, st);
   return hipGetLastError();
 }
 

@@ -33,7 +33,8 @@ hipError_t launch_vfilter(const Particles& P, const Sorted& S, const Grid& G, co
 // particles into the running totals (6 doubles), at the given time
 hipError_t launch_stats_fold(const double* part, int n, double* tot, double time, hipStream_t stream);
 // neighbour candidates of every sorted particle into L
-hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, hipStream_t stream);
+hipError_t launch_detect(const Sorted& S, const Grid& G, int n, const NbrList& L, const DevStatus* st,
+                         hipStream_t stream, int* mark = nullptr);
 // narrow phase of a system with non-spherical bodies: the contacts of every
 // sorted particle (candidates of L, or its 27 cells when L.count is null)
 // into sp.cf, and the contact statistics when sp.stats is set

@@ -145,13 +145,22 @@ int grains_gpu_system::allocate() {
     mem.cur = 0;
   }
   if (P.shape) {
+    // a record slot per listed candidate plus one per obstacle (up to 16)
+    // for the slot layout of the narrow phase, and at least kFeatCap
+    const size_t fcap = std::max<size_t>(kFeatCap, (size_t)std::max(nbr_cap, 0) +
+                                                       std::min<size_t>(obstacles.size(), 16));
     HIPCHECK(this, track(this, &cf.count, n));
-    HIPCHECK(this, track(this, &cf.ent, (size_t)kFeatCap * n));
-    HIPCHECK(this, track(this, &cf.pid, (size_t)kFeatCap * n));
-    for (int j = 0; j < 9; ++j) HIPCHECK(this, track(this, &cf.f[j], (size_t)kFeatCap * n));
+    HIPCHECK(this, track(this, &cf.ent, fcap * n));
+    HIPCHECK(this, track(this, &cf.pid, fcap * n));
+    for (int j = 0; j < 9; ++j) HIPCHECK(this, track(this, &cf.f[j], fcap * n));
     HIPCHECK(this, hipMemsetAsync(cf.count, 0, sizeof(int) * n, stream));
-    cf.cap = kFeatCap;
+    cf.cap = (int)fcap;
     cf.n = (int)n;
+    HIPCHECK(this, track(this, &cf.items, fcap * n));
+    HIPCHECK(this, track(this, &cf.nitems, 1));
+    int ncu = 0;
+    HIPCHECK(this, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+    cf.item_blocks = 8 * std::max(ncu, 1);  // one-wave workgroups, two per SIMD
   }
   HIPCHECK(this, track(this, &d_stat_part, 5 * ((n + 255) / 256) + 5));
   HIPCHECK(this, track(this, &d_stat_tot, 8));
@@ -385,7 +394,7 @@ int grains_gpu_system::forces(int mode, int first_step, bool commit) {
   const dem::NbrList L = nbr_list();
   if (L.count && !verlet) {
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
-    HIPCHECK(this, launch_detect(S, grid, nparticles, L, stream));
+    HIPCHECK(this, launch_detect(S, grid, nparticles, L, d_status, stream, P.shape ? cf.ent : nullptr));
     if (prof && (e = mark(prof->detect_ev, prof->ndetect))) return e;
   }
   if (P.shape) {
@@ -562,7 +571,7 @@ int grains_gpu_system::check_status() {
     }
     if (st.code == kContactOverflow) {
       snprintf(buf, sizeof(buf), "particle %d has %d contacts, more than the %d recorded per particle", st.id0,
-               st.id1, kFeatCap);
+               st.id1, cf.cap);
       return fail(this, GRAINS_SIMULATION_ERROR, buf);
     }
     snprintf(buf, sizeof(buf), "%s raised on the device (bodies %d and %d)", names[st.code & 3], st.id0, st.id1);

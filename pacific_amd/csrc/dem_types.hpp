@@ -174,17 +174,25 @@ enum ErrCode {
 };
 
 // Contacts of one step of a system with non-spherical bodies, as the narrow
-// phase (k_narrow) found them, for the force sweep: record j of sorted slot
-// k at [j * n + k] holds the partner (storage index >= 0, or -1 - obstacle
+// phase found them, for the force sweep: record j of sorted slot k at
+// [j * n + k] holds the partner (storage index >= 0, or -1 - obstacle
 // index), the partner's id, the partner's centre at its periodic image
 // (f[0..2]), the contact point (f[3..5]) and the overlap vector (f[6..8]) of
 // RigidBodyWithCrust::ClosestPoint, in the order the narrow phase met them.
+// A slot whose candidate is not in contact holds ent = kNoFeat (neither a
+// storage index nor an obstacle).
+constexpr int kNoFeat = -2147483647 - 1;
 struct ContactFeat {
-  int* count;    // [n] contacts of each sorted slot (may exceed cap: error)
+  int* count;    // [n] record slots of each sorted slot (may exceed cap: error)
   int* ent;      // [cap * n]
   int* pid;      // [cap * n]
   double* f[9];  // [cap * n] each
   int cap, n;
+  // the narrow phase's work list: (sorted slot << 32 | record slot) of
+  // every evaluation of the step, and its length
+  unsigned long long* items;  // [cap * n]
+  int* nitems;
+  int item_blocks;  // workgroups of the work-list kernel
 };
 enum Scheme { kLeapFrog = 0, kFirstOrderExplicit = 1, kSecondOrderExplicit = 2, kAdamsBashforth2 = 3 };
 

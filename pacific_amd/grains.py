@@ -501,6 +501,13 @@ class GrainsGPU:
         self._check(lib().grains_gpu_coordination_numbers(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return out
 
+    def coordination_at_state(self):
+        """Coordination numbers (id order) at the current state, from a force
+        evaluation whose torsor is then discarded (the run is unchanged)."""
+        out = np.zeros(self.n, dtype=np.int32)
+        self._check(lib().grains_gpu_coordination_at_state(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
+
     def write_rawdata(self, root, binary=False, truncate=False, coord=None):
         """One RawData frame at the current time (files <root>_*.dat)."""
         cp = None
