@@ -135,10 +135,12 @@ def pair_contacts(x, r, box=None):
 
 def batched_leg(args, prec):
     """512 independent settling boxes (sizes log-uniform in [200, 5000],
-    per-system mu and en) in one persistent launch per hipGraph replay.  The
-    ensemble settles untimed until its beds form, the autotuner picks the
-    workgroup size on that contact-bearing state, then the settling run is
-    timed; contacts per particle from the member states at the end."""
+    per-system mu and en), each hipGraph replay a chunk of steps of the
+    batch.  The ensemble settles untimed until every layer has landed and the
+    beds form, the autotuner picks the launch shape (flat mode, or the
+    persistent launch and its workgroup size) on that contact-bearing state,
+    then 5,000 steps are timed; contacts per particle from the member states
+    at the end."""
     from pacific_amd import GrainsBatch, GrainsGPU, workloads
     ens = workloads.ensemble(512, seed=5)
     members = []
@@ -149,8 +151,12 @@ def batched_leg(args, prec):
         members.append(g)
     b = GrainsBatch(members, precision=prec)
     del members
-    chunk = 100
-    b.step(args.batch_settle, chunk=chunk)
+    chunk = args.batch_chunk
+    t0 = time.perf_counter()
+    for done in range(0, args.batch_settle, 5000):
+        b.step(min(5000, args.batch_settle - done), chunk=chunk)
+        log(f"bench: ensemble settle {min(done + 5000, args.batch_settle)}/{args.batch_settle} steps "
+            f"({time.perf_counter() - t0:.1f} s)")
     best, tune = b.autotune(20)
     ms = b.step_timed(args.batch_steps, chunk=chunk)
     ntot = b.n_total
@@ -159,15 +165,38 @@ def batched_leg(args, prec):
     for i, wl in enumerate(ens):
         ncont += pair_contacts(b.state(i)["x"], np.asarray(wl["radius"], float) * np.ones(len(wl["x"])))
     cbar = ncont / ntot
-    t_end = args.batch_settle + 8 * 20 + args.batch_steps
-    log(f"bench: batched ensemble 512 systems, {ntot} particles, workgroup {best}: {value:.4g} particle-steps/s "
+    t_end = args.batch_settle + len(tune) * 20 + args.batch_steps
+    shape = "flat" if best == 0 else f"persistent, workgroup {best}"
+    log(f"bench: batch tuner (ms per {20} steps): " + ", ".join(f"{'flat' if k == 0 else k}: {v:.2f}"
+                                                             for k, v in tune.items()))
+    log(f"bench: batched ensemble 512 systems, {ntot} particles, {shape}: {value:.4g} particle-steps/s "
         f"(steps {t_end - args.batch_steps}-{t_end}, particle contacts/particle {cbar:.3f} at the end)")
+    # one system of the same density and history for comparison: the
+    # ensemble's tallest box (26 lattice layers) widened to 200,000 spheres,
+    # default search (Verlet list), same untimed settle, 2,000 steps timed
+    from pacific_amd import FP64
+    side = int(np.ceil(np.sqrt(200_000 / 26)))
+    wl1 = workloads.settling_box(200_000, seed=77, box=(side * 2.5e-3, side * 2.5e-3, 26 * 3.75e-3))
+    g1 = build_gpu(wl1, FP64 if prec == 0 else prec)
+    g1.step(args.batch_settle)
+    g1.synchronize()
+    ms1 = g1.step_timed(2000)
+    single = 200_000 * 2000 / (ms1 / 1e3)
+    c1 = contacts_per_particle(g1)
+    del g1
+    log(f"bench: one dense box of 200,000 spheres after the same settle: {single:.4g} particle-steps/s "
+        f"(particle contacts/particle {c1:.3f}); batched / single {value / single:.3f}")
     return {"batched_value": value,
+            "batched_single_dense_value": single,
+            "batched_single_dense_config": {"particles": 200_000, "box_mm": [side * 2.5, side * 2.5, 26 * 3.75],
+                                            "settle_steps": args.batch_settle, "timed_steps": 2000,
+                                            "particle_contacts_per_particle": c1, "search": "Verlet list"},
             "batched_config": {"systems": 512, "particles": ntot, "precision": "fp64" if prec == 0 else "mixed",
                                "untimed_settle_steps": args.batch_settle, "autotune_trial_steps": 20,
                                "timed_steps": args.batch_steps, "timed_window": [t_end - args.batch_steps, t_end],
-                               "particle_contacts_per_particle_at_end": cbar, "workgroup": best,
-                               "graph_chunk": chunk, "autotune_ms": {str(k): v for k, v in tune.items()}}}
+                               "particle_contacts_per_particle_at_end": cbar, "launch_shape": shape,
+                               "graph_chunk": chunk,
+                               "autotune_ms": {("flat" if k == 0 else str(k)): v for k, v in tune.items()}}}
 
 
 def kinetic_energy(g, m, I):
@@ -472,9 +501,11 @@ def main():
     ap.add_argument("--skin", type=float, default=-1.0,
                     help="Verlet skin in m (< 0: a tenth of the cell size; 0: candidate list rebuilt every step)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
-    ap.add_argument("--batch-steps", type=int, default=3000)
-    ap.add_argument("--batch-settle", type=int, default=2000,
-                    help="untimed ensemble steps before the tuner (the beds form)")
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--batch-chunk", type=int, default=100,
+                    help="steps per captured hipGraph of the batch (0: direct launches, for tracers)")
+    ap.add_argument("--batch-settle", type=int, default=15000,
+                    help="untimed ensemble steps before the tuner (every layer has landed, the beds form)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost-ratio leg")
     ap.add_argument("--no-poly", action="store_true", help="skip the polyhedra (GJK) leg")
     ap.add_argument("--poly-n", type=int, default=200_000)

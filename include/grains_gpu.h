@@ -354,14 +354,17 @@ typedef struct grains_gpu_batch grains_gpu_batch;
  * member and computes its initial forces. */
 int grains_gpu_batch_create(grains_gpu_system* const* systems, int nsys, int precision, grains_gpu_batch** out);
 void grains_gpu_batch_destroy(grains_gpu_batch* b);
-/* workgroup size per system: a multiple of 64 from 64 to 512 */
+/* launch shape: 0 = flat mode (one launch per phase over the particles of
+ * every member), or the workgroup size per member of the persistent launch,
+ * a multiple of 64 from 64 to 512 */
 int grains_gpu_batch_set_block(grains_gpu_batch* b, int block);
 /* nsteps steps of every member; whole chunks of `chunk` steps replay one
- * captured hipGraph of the persistent launch (chunk <= 0: direct launches) */
+ * captured hipGraph of the step's launches (chunk <= 0: direct launches) */
 int grains_gpu_batch_step(grains_gpu_batch* b, int nsteps, int chunk);
 int grains_gpu_batch_step_timed(grains_gpu_batch* b, int nsteps, int chunk, double* ms);
-/* times trial_steps steps at each workgroup size (64, 128, 192, ..., 512 ->
- * ms_out[0..7]), keeps the fastest; advances the members by 8 * trial_steps */
+/* times trial_steps steps in each launch shape (0 = flat, then workgroups
+ * of 64, 128, 192, ..., 512 -> ms_out[0..8]), keeps the fastest; advances
+ * the members by 9 * trial_steps */
 int grains_gpu_batch_autotune(grains_gpu_batch* b, int trial_steps, int* best_block, double* ms_out);
 /* particles of member isys, or of the whole batch for isys < 0 */
 size_t grains_gpu_batch_num_particles(const grains_gpu_batch* b, int isys);

@@ -1,8 +1,11 @@
 // Batched ensembles: many independent DEM systems (unevenly sized, each with
-// its own box, obstacles and contact parameters) advanced by ONE persistent
-// launch.  Workgroup b owns system sys_order[b] and runs every phase of every
-// step for it, separated by workgroup barriers only: no workgroup ever waits
-// on another, so the launch cannot deadlock whatever the residency.
+// its own box, obstacles and contact parameters) advanced together, in one
+// of two launch shapes the autotuner chooses between.
+//
+// Persistent mode (workgroup size 64..512): ONE launch; workgroup b owns
+// system sys_order[b] and runs every phase of every step for it, separated by
+// workgroup barriers only: no workgroup ever waits on another, so the launch
+// cannot deadlock whatever the residency.
 //
 // The phases reuse the single-system device functions (dem_device.hpp), in
 // the same order, so system i of a batch evolves bit for bit like system i
@@ -14,6 +17,20 @@
 //   second half kick
 // The reference runs one system per process (GrainsExec, ContactBuilderFactory
 // and Cell hold static state); the batch is this build's own capability.
+//
+// Flat mode (workgroup size 0, the default): the same phases as one launch
+// each over the particles of every system at once -- each thread reads its
+// system's grid, obstacles and models from the system table -- with the cell
+// counts of all systems scanned as one array (the systems' cells and
+// particles are laid out in the same order, so a global exclusive scan is
+// each system's own start table offset by its first particle), and a
+// per-step candidate list (k_fb_detect) that the force sweep reads.  Large
+// systems then spread over the whole chip instead of one workgroup's CU, at
+// the price of a few launches per step; the forces, the second half kick
+// and the next drift share one kernel, velocities double buffered (the sweep
+// reads the neighbours' while each thread writes its own).  Same device
+// functions, same contact order: members evolve bit for bit as in the
+// persistent mode.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,9 +41,14 @@
 
 #include "../../include/grains_gpu.h"
 #include "dem_device.hpp"
+#include "dem_launch.hpp"
 #include "dem_system.hpp"
 
 namespace dem {
+
+// candidates per particle of the flat mode's per-step list (a particle with
+// more is searched in its 27 cells by the force sweep)
+constexpr int kBatchNbrCap = 32;
 
 struct BatchSys {
   Grid grid;
@@ -63,6 +85,28 @@ __device__ void wg_scan(const int* __restrict__ count, int* __restrict__ start, 
     carry += total;
   }
   if (t == 0) start[s0 + nc] = carry;
+}
+
+__device__ __forceinline__ StepParams batch_params(const BatchSys& B, int hi) {
+  StepParams sp;
+  sp.g[0] = B.g[0];
+  sp.g[1] = B.g[1];
+  sp.g[2] = B.g[2];
+  sp.dt = B.dt;
+  sp.n = hi;
+  sp.cur = 0;
+  sp.nobs = B.nobs;
+  sp.nmat = kMaxMaterials;
+  sp.record_pairs = 0;
+  sp.pair_cap = 0;
+  sp.scheme = kLeapFrog;
+  sp.first_step = 0;
+  sp.stats = nullptr;
+  sp.mem = ContactMemory{};
+  sp.cf = ContactFeat{};
+  sp.fsum = nullptr;
+  sp.hyd = nullptr;
+  return sp;
 }
 
 // mode 0: bin the current positions and compute the initial forces (LeapFrog
@@ -175,6 +219,136 @@ __global__ __launch_bounds__(512) void k_batch(const BatchSys* __restrict__ sys,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Flat mode kernels.  sys holds the system table with s0 = c0 (one start
+// table for all cells); sysid the system of each storage index.
+
+// move = true: first half kick and drift from the stored torsor (the first
+// step of a batch); false: the key of the current positions only (priming).
+__global__ __launch_bounds__(256) void k_fb_move(const BatchSys* __restrict__ sys, const int* __restrict__ sysid,
+                                                 Particles P, Sorted S, int n, bool move, DevStatus* st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const BatchSys& B = sys[sysid[s]];
+  Kin k = load_kin(P, s, 0, 0);
+  const int pid = P.id[s];
+  if (move) {
+    const double depl = lf_move(k, ld3(P.F, s), ld3(P.T, s), P.mass[s], P.cfac[s], P.inertia[s], 0.5 * B.dt, B.dt);
+    if (depl > P.crust[s]) raise_error(st, kMotionError, pid, 0);
+  }
+  wrap_and_bin(k, B.grid, S, st, s, pid, B.c0);
+  st3(P.x[0], s, k.x);
+  st3(P.v[0], s, k.v);
+  st3(P.w[0], s, k.w);
+  st3(P.wb, s, k.wb);
+  P.q[0][s] = k.q.v.x;
+  P.q[1][s] = k.q.v.y;
+  P.q[2][s] = k.q.v.z;
+  P.q[3][s] = k.q.w;
+}
+
+// One thread per sorted slot: the candidates of the particle in its
+// system's 27 cells that pass the narrow phase's conservative distance
+// reject, in the order of the cell sweep (k_detect with the system's grid).
+__global__ __launch_bounds__(256) void k_fb_detect(const BatchSys* __restrict__ sys, const int* __restrict__ sysid,
+                                                   Sorted S, int n, NbrList L) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int s = S.order[k];
+  if ((unsigned)s >= (unsigned)n) {
+    L.count[k] = 0;
+    return;
+  }
+  const BatchSys& B = sys[sysid[s]];
+  const double* pr = S.posr + 4 * (size_t)k;
+  const d3 x{pr[0], pr[1], pr[2]};
+  const double r = pr[3];
+  int c = 0;
+  for_each_candidate(S, B.grid, B.s0, B.p0, B.p0 + B.np, x, [&](int k2, int ix, int iy, int iz) {
+    if (k2 == k) return;
+    const double* q2 = S.posr + 4 * (size_t)k2;
+    const d3 xo{ix ? q2[0] + image_shift(B.grid, 0, ix) : q2[0], iy ? q2[1] + image_shift(B.grid, 1, iy) : q2[1],
+                iz ? q2[2] + image_shift(B.grid, 2, iz) : q2[2]};
+    const d3 AB = xo - x;
+    const double s2 = AB.x * AB.x + AB.y * AB.y + AB.z * AB.z;
+    const double R = r + q2[3];
+    if (s2 > R * R * 1.000000000001) return;
+    if (c < L.cap) L.ent[(size_t)c * n + k] = nbr_entry(k2, ix, iy, iz);
+    ++c;
+  });
+  L.count[k] = c;
+}
+
+// One thread per sorted slot: the particle's contacts (its candidates of L,
+// or its system's 27 cells when the candidates overflowed the list) and
+// obstacles, then MODE 0 the torsor only (priming), 1 the second
+// half kick (last step of a batch; torsor stored), 2 the second half kick,
+// the next first half kick and drift and the next cell key.  Velocities are
+// read from buffer cv and written to cv ^ 1.
+// (held to three waves per SIMD: the FP64 variant would take two)
+template <bool MIXED, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_fb_forces(const BatchSys* __restrict__ sys, const int* __restrict__ sysid,
+                                                   Particles P, Sorted S, const BoxObstacle* __restrict__ obs,
+                                                   const HookeModel* __restrict__ models, int n, int cv,
+                                                   DevStatus* st, NbrList L) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int s = S.order[k];
+  if ((unsigned)s >= (unsigned)n) return;
+  const BatchSys& B = sys[sysid[s]];
+  const int lo = B.p0, hi = B.p0 + B.np;
+  const StepParams sp = batch_params(B, hi);
+  const double* pr = S.posr + 4 * (size_t)k;
+  Self me;
+  me.x = d3{pr[0], pr[1], pr[2]};
+  me.r = pr[3];
+  me.id = S.sid[k];
+  me.v = ld3(P.v[cv], s);
+  me.w = ld3(P.w[cv], s);
+  me.m = P.mass[s];
+  me.c = P.crust[s];
+  me.req = P.req[s];
+  me.mat = P.mat[s];
+  const double wf = P.wfac[s];
+  d3 F{B.g[0] * wf, B.g[1] * wf, B.g[2] * wf};
+  d3 T{0., 0., 0.};
+  sweep_particle<MIXED>(P, S, B.grid, obs + B.o0, B.nobs, models + B.m0, sp, st, nullptr, B.s0, lo, hi, cv, k, me, F,
+                        T, nullptr, 0, &L);
+  if (MODE != 2) {
+    st3(P.F, s, F);
+    st3(P.T, s, T);
+  }
+  if (MODE == 0) return;
+  Kin kin;
+  kin.x = me.x;
+  kin.v = me.v;
+  kin.w = me.w;
+  kin.wb = ld3(P.wb, s);
+  kin.q = quat{d3{P.q[0][s], P.q[1][s], P.q[2][s]}, P.q[3][s]};
+  const double cf = P.cfac[s], I = P.inertia[s];
+  lf_kick(kin, F, T, me.m, cf, I, 0.5 * B.dt);
+  if (MODE == 2) {
+    const double depl = lf_move(kin, F, T, me.m, cf, I, 0.5 * B.dt, B.dt);
+    if (depl > me.c) raise_error(st, kMotionError, me.id, 0);
+    wrap_and_bin(kin, B.grid, S, st, s, me.id, B.c0);
+    st3(P.x[0], s, kin.x);
+    P.q[0][s] = kin.q.v.x;
+    P.q[1][s] = kin.q.v.y;
+    P.q[2][s] = kin.q.v.z;
+    P.q[3][s] = kin.q.w;
+  }
+  st3(P.v[cv ^ 1], s, kin.v);
+  st3(P.w[cv ^ 1], s, kin.w);
+  st3(P.wb, s, kin.wb);
+}
+
+__global__ __launch_bounds__(256) void k_fb_copy_v(Particles P, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  st3(P.v[0], s, ld3(P.v[1], s));
+  st3(P.w[0], s, ld3(P.w[1], s));
+}
+
 }  // namespace dem
 
 using namespace dem;
@@ -184,7 +358,7 @@ struct grains_gpu_batch {
   int ntot = 0;
   int precision = GRAINS_FP64;
   int device = 0;
-  int block = 256;
+  int block = 0;  // flat mode; 64..512: the persistent launch's workgroup size
   int graph_steps = 0;
   double time = 0.;
   std::vector<int> np, p0;
@@ -196,6 +370,14 @@ struct grains_gpu_batch {
   Sorted S{};
   BatchSys* d_sys = nullptr;
   int* d_order = nullptr;
+  // flat mode: system table with one start table, system of each particle,
+  // sorted arrays over that start table, second velocity buffers
+  BatchSys* d_fsys = nullptr;
+  int* d_sysid = nullptr;
+  Sorted Sf{};
+  int ncells = 0;
+  int* d_nbr_count = nullptr;  // per-step candidate list of the flat mode
+  unsigned* d_nbr = nullptr;
   BoxObstacle* d_obs = nullptr;
   HookeModel* d_models = nullptr;
   DevStatus* d_status = nullptr;
@@ -206,6 +388,7 @@ struct grains_gpu_batch {
   std::vector<void*> allocations;
   ~grains_gpu_batch();
   int launch(int mode, int nsteps);
+  int launch_flat(int mode, int nsteps);
   int check();
 };
 
@@ -248,7 +431,52 @@ grains_gpu_batch::~grains_gpu_batch() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+int grains_gpu_batch::launch_flat(int mode, int nsteps) {
+  const int n = ntot;
+  if (n == 0) return GRAINS_OK;
+  const dim3 grid((n + 255) / 256), blk(256);
+  Grid all{};
+  all.ncells = ncells;
+  const bool mixed = precision == GRAINS_MIXED;
+  const NbrList L{d_nbr_count, d_nbr, kBatchNbrCap, n};
+  auto detect = [&]() {
+    hipLaunchKernelGGL(k_fb_detect, grid, blk, 0, stream, d_fsys, d_sysid, Sf, n, L);
+  };
+  auto forces = [&](int m, int cv) {
+    if (mixed) {
+      if (m == 0) hipLaunchKernelGGL((k_fb_forces<true, 0>), grid, blk, 0, stream, d_fsys, d_sysid, P, Sf, d_obs, d_models, n, cv, d_status, L);
+      else if (m == 1) hipLaunchKernelGGL((k_fb_forces<true, 1>), grid, blk, 0, stream, d_fsys, d_sysid, P, Sf, d_obs, d_models, n, cv, d_status, L);
+      else hipLaunchKernelGGL((k_fb_forces<true, 2>), grid, blk, 0, stream, d_fsys, d_sysid, P, Sf, d_obs, d_models, n, cv, d_status, L);
+    } else {
+      if (m == 0) hipLaunchKernelGGL((k_fb_forces<false, 0>), grid, blk, 0, stream, d_fsys, d_sysid, P, Sf, d_obs, d_models, n, cv, d_status, L);
+      else if (m == 1) hipLaunchKernelGGL((k_fb_forces<false, 1>), grid, blk, 0, stream, d_fsys, d_sysid, P, Sf, d_obs, d_models, n, cv, d_status, L);
+      else hipLaunchKernelGGL((k_fb_forces<false, 2>), grid, blk, 0, stream, d_fsys, d_sysid, P, Sf, d_obs, d_models, n, cv, d_status, L);
+    }
+  };
+  hipLaunchKernelGGL(k_fb_move, grid, blk, 0, stream, d_fsys, d_sysid, P, Sf, n, mode != 0, d_status);
+  BCHECK(this, launch_bin(P, Sf, all, n, 0, stream));
+  detect();
+  if (mode == 0) {
+    forces(0, 0);
+    BCHECK(this, hipGetLastError());
+    return GRAINS_OK;
+  }
+  int cv = 0;
+  for (int j = 1; j < nsteps; ++j) {
+    forces(2, cv);
+    cv ^= 1;
+    BCHECK(this, launch_bin(P, Sf, all, n, 0, stream));
+    detect();
+  }
+  forces(1, cv);
+  cv ^= 1;
+  if (cv) hipLaunchKernelGGL(k_fb_copy_v, grid, blk, 0, stream, P, n);
+  BCHECK(this, hipGetLastError());
+  return GRAINS_OK;
+}
+
 int grains_gpu_batch::launch(int mode, int nsteps) {
+  if (block == 0) return launch_flat(mode, nsteps);
   size_t lds = sizeof(int) * (size_t)block;
   if (precision == GRAINS_MIXED)
     hipLaunchKernelGGL(k_batch<true>, dim3(nsys), dim3(block), lds, stream, d_sys, d_order, P, S, d_obs, d_models,
@@ -409,6 +637,29 @@ int grains_gpu_batch_create(grains_gpu_system* const* systems, int nsys, int pre
   BCHECK(b, hipMemset(S.order, 0, sizeof(int) * n));
   BCHECK(b, balloc(b, &b->d_sys, (size_t)nsys));
   BCHECK(b, up(b->d_sys, hs));
+  // flat mode
+  {
+    std::vector<BatchSys> fs = hs;
+    for (auto& B : fs) B.s0 = B.c0;
+    BCHECK(b, balloc(b, &b->d_fsys, (size_t)nsys));
+    BCHECK(b, up(b->d_fsys, fs));
+    std::vector<int> sid(n);
+    for (int i = 0; i < nsys; ++i)
+      for (int p = 0; p < b->np[i]; ++p) sid[(size_t)b->p0[i] + p] = i;
+    BCHECK(b, balloc(b, &b->d_sysid, n));
+    BCHECK(b, up(b->d_sysid, sid));
+    b->ncells = (int)cells;
+    b->Sf = S;
+    BCHECK(b, balloc(b, &b->Sf.start, (size_t)cells + 1));
+    BCHECK(b, hipMemset(b->Sf.start, 0, sizeof(int) * ((size_t)cells + 1)));
+    BCHECK(b, balloc(b, &b->Sf.block_sums, (size_t)scan_blocks((int)cells) + 1));
+    for (int a = 0; a < 3; ++a) {
+      BCHECK(b, balloc(b, &P.v[1][a], n));
+      BCHECK(b, balloc(b, &P.w[1][a], n));
+    }
+    BCHECK(b, balloc(b, &b->d_nbr_count, n));
+    BCHECK(b, balloc(b, &b->d_nbr, (size_t)kBatchNbrCap * n));
+  }
   BCHECK(b, balloc(b, &b->d_order, (size_t)nsys));
   BCHECK(b, up(b->d_order, order));
   BCHECK(b, balloc(b, &b->d_obs, hobs.size()));
@@ -426,7 +677,7 @@ int grains_gpu_batch_create(grains_gpu_system* const* systems, int nsys, int pre
 void grains_gpu_batch_destroy(grains_gpu_batch* b) { delete b; }
 
 int grains_gpu_batch_set_block(grains_gpu_batch* b, int block) {
-  if (!b || block < 64 || block > 512 || block % 64) return GRAINS_ARGUMENT_ERROR;
+  if (!b || (block != 0 && (block < 64 || block > 512 || block % 64))) return GRAINS_ARGUMENT_ERROR;
   if (block != b->block && b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
@@ -509,18 +760,19 @@ int grains_gpu_batch_step_timed(grains_gpu_batch* b, int nsteps, int chunk, doub
   return b->check();
 }
 
-// Times `trial_steps` steps at every workgroup size and keeps the fastest;
-// ms_out[i] receives the time of candidate i (64, 128, ..., 512: whole
-// waves, so that sizes between the powers of two can fill the wave slots a
-// CU has at the kernel's register count).
+// Times `trial_steps` steps in every launch shape and keeps the fastest;
+// ms_out[i] receives the time of candidate i: 0 (flat mode), then the
+// persistent launch with workgroups of 64, 128, ..., 512 lanes (whole waves,
+// so that sizes between the powers of two can fill the wave slots a CU has
+// at the kernel's register count).
 int grains_gpu_batch_autotune(grains_gpu_batch* b, int trial_steps, int* best_block, double* ms_out) {
   if (!b || trial_steps <= 0) return GRAINS_ARGUMENT_ERROR;
   BCHECK(b, hipSetDevice(b->device));
-  constexpr int kCand = 8;
+  constexpr int kCand = 9;
   double best = 1e300;
   int bestb = b->block;
   for (int i = 0; i < kCand; ++i) {
-    const int cand[kCand] = {64, 128, 192, 256, 320, 384, 448, 512};
+    const int cand[kCand] = {0, 64, 128, 192, 256, 320, 384, 448, 512};
     grains_gpu_batch_set_block(b, cand[i]);
     BCHECK(b, hipEventRecord(b->ev0, b->stream));
     int e = b->launch(1, trial_steps);

@@ -1061,10 +1061,11 @@ template <class V, bool SHAPES = false>
 __device__ __forceinline__ void for_each_listed(const Particles& P, const Sorted& S, const Grid& G,
                                                 const BoxObstacle* __restrict__ obs, int nobs, DevStatus* st,
                                                 int slot_lo, int slot_hi, int k, const Self& me, V& vis,
-                                                const NbrList& L, const ConvexGeo* mg = nullptr, int cx = 0) {
+                                                const NbrList& L, const ConvexGeo* mg = nullptr, int cx = 0,
+                                                int start_base = 0) {
   const int c = L.count[k];
   if (c > L.cap) {
-    for_each_contact<V, SHAPES>(P, S, G, obs, nobs, st, 0, slot_lo, slot_hi, k, me, vis, mg, cx);
+    for_each_contact<V, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis, mg, cx);
     return;
   }
   for (int i = 0; i < c; ++i) {
@@ -1255,7 +1256,7 @@ __device__ __forceinline__ void sweep_particle(const Particles& P, const Sorted&
     for_each_verlet<FV, SHAPES>(P, S, G, *VS, obs, nobs, st, slot_hi, s, me, vis, *L, mg, cx,
                                 CL && CL->count ? CL : nullptr);
   else if (L && L->count)
-    for_each_listed<FV, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx);
+    for_each_listed<FV, SHAPES>(P, S, G, obs, nobs, st, slot_lo, slot_hi, k, me, vis, *L, mg, cx, start_base);
   else
     for_each_contact<FV, SHAPES>(P, S, G, obs, nobs, st, start_base, slot_lo, slot_hi, k, me, vis, mg, cx);
   F = vis.F;

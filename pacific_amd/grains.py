@@ -567,6 +567,11 @@ class GrainsGPU:
         return out
 
 
+# launch shapes of a batch: 0 = flat mode, else the persistent launch's
+# workgroup size per member
+BATCH_SHAPES = (0, 64, 128, 192, 256, 320, 384, 448, 512)
+
+
 class GrainsBatch:
     """Independent systems advanced together by one persistent launch.
 
@@ -613,9 +618,9 @@ class GrainsBatch:
 
     def autotune(self, trial_steps=20):
         best = ctypes.c_int()
-        ms = np.zeros(8)
+        ms = np.zeros(9)
         self._check(lib().grains_gpu_batch_autotune(self.h, int(trial_steps), ctypes.byref(best), _d(ms)))
-        return best.value, dict(zip((64, 128, 192, 256, 320, 384, 448, 512), ms.tolist()))
+        return best.value, dict(zip(BATCH_SHAPES, ms.tolist()))
 
     def state(self, i):
         n = self.sizes[i]

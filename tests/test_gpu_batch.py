@@ -1,6 +1,8 @@
-"""Batched ensembles (one persistent launch, one workgroup per system): every
-member evolves bit for bit like the same system stepped on its own, through
-direct launches and through replays of the captured hipGraph."""
+"""Batched ensembles (flat mode, the default: one launch per phase over the
+particles of every member; persistent mode: one launch, one workgroup per
+member): every member evolves bit for bit like the same system stepped on
+its own, in both modes, through direct launches and through replays of the
+captured hipGraph."""
 import numpy as np
 import pytest
 
@@ -24,9 +26,11 @@ def unconfigured(wl, precision=0):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_members_match_single_systems(gpu, precision):
+@pytest.mark.parametrize("shape", [0, 256])
+def test_members_match_single_systems(gpu, precision, shape):
     ens = members()
     batch = GrainsBatch([unconfigured(w, precision) for w in ens], precision=precision)
+    batch.set_block(shape)
     batch.step(1500, chunk=500)
     for i, wl in enumerate(ens):
         g = make_gpu(wl, precision=precision, record=False)
@@ -36,10 +40,13 @@ def test_members_match_single_systems(gpu, precision):
             assert np.array_equal(sb[k], sg[k]), (i, k)
 
 
-def test_graph_replay_equals_direct_launch(gpu):
+@pytest.mark.parametrize("shape", [0, 256])
+def test_graph_replay_equals_direct_launch(gpu, shape):
     ens = members(4, seed=8)
     b1 = GrainsBatch([unconfigured(w) for w in ens])
     b2 = GrainsBatch([unconfigured(w) for w in ens])
+    b1.set_block(shape)
+    b2.set_block(shape)
     b1.step(600, chunk=0)
     b2.step(600, chunk=150)
     for i in range(len(ens)):
@@ -61,6 +68,30 @@ def test_member_against_oracle(gpu):
 def test_autotune_picks_a_candidate(gpu):
     batch = GrainsBatch([unconfigured(w) for w in members(8, seed=2)])
     best, ms = batch.autotune(10)
-    assert best in (64, 128, 192, 256, 320, 384, 448, 512) and len(ms) == 8
+    assert best in (0, 64, 128, 192, 256, 320, 384, 448, 512) and len(ms) == 9
     assert all(t > 0 for t in ms.values())
     assert ms[best] == min(ms.values())
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_flat_mode_matches_persistent_and_single(gpu, precision):
+    # the flat launches run the persistent kernel's device functions in the
+    # same order: members bit for bit as there and as the system alone,
+    # through direct launches and graph replays, with an odd step count (the
+    # velocity buffers end swapped and are copied back)
+    ens = members(5, seed=4)
+    bf = GrainsBatch([unconfigured(w, precision) for w in ens], precision=precision)
+    bp = GrainsBatch([unconfigured(w, precision) for w in ens], precision=precision)
+    bf.set_block(0)
+    bp.set_block(256)
+    bf.step(1201, chunk=400)
+    bp.step(1201, chunk=400)
+    bf.step(3)
+    bp.step(3)
+    for i, wl in enumerate(ens):
+        g = make_gpu(wl, precision=precision, record=False, nbr=0)
+        g.step(1204)
+        sf, sp, sg = bf.state(i), bp.state(i), g.state()
+        for k in ("x", "v", "w", "q", "F", "T"):
+            assert np.array_equal(sf[k], sp[k]), (i, k)
+            assert np.array_equal(sf[k], sg[k]), (i, k)
