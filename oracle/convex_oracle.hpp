@@ -223,16 +223,37 @@ inline bool jDegenerate(Simplex const& S, unsigned all_bits, V3 const& w) {
   return false;
 }
 
-// closest_points (Convex.cpp:536-627) without Nesterov acceleration (the
-// default), relative tolerance 1e-10 (GrainsExec::m_colDetTolerance)
+// Options of <CollisionDetectionAlgorithm> (Grains.cpp:640-770;
+// GrainsExec::m_colDetGJK_SV, m_colDetTolerance, m_colDetAcceleration,
+// m_colDetBoundingVolume), set per system and installed here by each oracle
+// entry point that evaluates contacts.
+struct CollisionOptions {
+  int sv = 0;         // 1: signed-volume sub-distance (GJK_SV.cpp)
+  double tol = 1e-10; // relative tolerance of the optimality gap
+  int accel = 0;      // 1: Nesterov acceleration
+  int bv = 0;         // bounding-volume pre-test: 0 none, 1 OBB, 2 OBC
+};
+CollisionOptions g_col;
+
+// Nesterov direction of closest_points (Convex.cpp:575-585, the same in
+// GJK_SV.cpp:479-489)
+inline V3 nesterovDirection(int it, V3 const& d, V3 const& v, V3 const& w) {
+  double momentum = it / (it + 2.), omm = 1. - momentum;
+  return momentum * d + momentum * omm * v + omm * omm * w;
+}
+
+// closest_points (Convex.cpp:536-627): Johnson's sub-distance, relative
+// tolerance and acceleration of g_col
 double gjkClosestPoints(ConvexRef const& a, ConvexRef const& b, Tf const& a2w, Tf const& b2w, V3& pa, V3& pb,
                         int& nbIter) {
   unsigned bits = 0, last = 0, last_bit = 0, all_bits = 0;
   Simplex S;
   int it = 0;
-  const double relError = EPSILON, absError = 1.e-4 * relError;
+  const double relError = g_col.tol, absError = 1.e-4 * relError;
+  bool acceleration = g_col.accel != 0;
   V3 v = applyTf(a2w, support(a, V3())) - applyTf(b2w, support(b, V3()));
   V3 w = v;
+  V3 d = v;
   double dist = Norm(v);
   while (bits < 15 && dist > EPSILON2 && it < 1000) {
     ++it;
@@ -242,12 +263,20 @@ double gjkClosestPoints(ConvexRef const& a, ConvexRef const& b, Tf const& a2w, T
       ++last;
       last_bit <<= 1;
     }
-    V3 d = v;
+    if (acceleration && it % 8 != 0) d = nesterovDirection(it, d, v, w);
+    else d = v;
     S.p[last] = support(a, vecTimesBasis(-d, a2w));
     S.q[last] = support(b, vecTimesBasis(d, b2w));
     w = applyTf(a2w, S.p[last]) - applyTf(b2w, S.q[last]);
     double mu = dist - dot(v, w) / dist;
-    if (mu < dist * relError || mu < absError) break;
+    if (mu < dist * relError || mu < absError) {
+      if (!acceleration) break;
+      if (Norm(d - v) < EPSILON) break;
+      acceleration = false;
+      S.p[last] = support(a, vecTimesBasis(-v, a2w));
+      S.q[last] = support(b, vecTimesBasis(v, b2w));
+      w = applyTf(a2w, S.p[last]) - applyTf(b2w, S.q[last]);
+    }
     if (jDegenerate(S, all_bits, w)) break;
     S.y[last] = w;
     all_bits = bits | last_bit;
@@ -257,6 +286,404 @@ double gjkClosestPoints(ConvexRef const& a, ConvexRef const& b, Tf const& a2w, T
   jPoints(S, bits, pa, pb);
   nbIter = it;
   return dist;
+}
+
+// --- Signed-volume sub-distance (GJK_SV.cpp:1-420) --------------------------
+// The subalgorithm of Montanari, Petrinic and Barbieri (2017) as the
+// reference writes it: points selected by the bits of the simplex, signed
+// volumes of the sub-simplices in the projection of largest area, and the
+// recursion into the faces (edges) whose signed volume disagrees, keeping the
+// closest.
+inline bool svSameSign(double a, double b) { return !((a > 0) ^ (b > 0)); }
+
+inline void svS1d(V3 const y[4], unsigned& bits, double (&lam)[4]) {
+  bool s1_set = false;
+  unsigned i1 = 0xffffffff, i2 = 0xffffffff;
+  for (unsigned i = 0; i < 4; ++i)
+    if (bits & (1u << i)) {
+      if (s1_set) {
+        i2 = i;
+        break;
+      }
+      i1 = i;
+      s1_set = true;
+    }
+  V3 t = y[i2] - y[i1];
+  unsigned I = 0;
+  double neg_tI = -t[0];
+  if (fabs(t[1]) > fabs(neg_tI)) {
+    I = 1;
+    neg_tI = -t[1];
+  }
+  if (fabs(t[2]) > fabs(neg_tI)) {
+    I = 2;
+    neg_tI = -t[2];
+  }
+  double pI = dot(y[i2], t) / dot(t, t) * neg_tI + y[i2][I];
+  double C[2] = {-y[i2][I] + pI, y[i1][I] - pI};
+  bool sc0 = svSameSign(neg_tI, C[0]), sc1 = svSameSign(neg_tI, C[1]);
+  if (sc0 && sc1) {
+    lam[i1] = C[0] / neg_tI;
+    lam[i2] = C[1] / neg_tI;
+  } else if (sc0) {
+    bits &= ~(1u << i2);
+    lam[i1] = 1.;
+  } else {
+    bits &= ~(1u << i1);
+    lam[i2] = 1.;
+  }
+}
+
+inline void svS2d(V3 const y[4], unsigned& bits, double (&lam)[4]) {
+  unsigned counter = 0, i0 = 0, i1 = 0, i2 = 0;
+  for (unsigned i = 0; i < 4; ++i)
+    if (bits & (1u << i)) {
+      if (counter == 0) i0 = i;
+      else if (counter == 1) i1 = i;
+      else i2 = i;
+      counter += 1;
+    }
+  V3 n = cross(y[i1] - y[i0], y[i2] - y[i0]);
+  V3 p0 = (dot(y[i0], n) / dot(n, n)) * n;
+  unsigned ix = 1, iy = 2;
+  double mu_max = (y[i1][1] * y[i2][2] + y[i0][1] * y[i1][2] + y[i2][1] * y[i0][2] - y[i1][1] * y[i0][2] -
+                   y[i2][1] * y[i1][2] - y[i0][1] * y[i2][2]);
+  double mu = (y[i1][2] * y[i0][0] + y[i2][2] * y[i1][0] + y[i0][2] * y[i2][0] - y[i1][2] * y[i2][0] -
+               y[i0][2] * y[i1][0] - y[i2][2] * y[i0][0]);
+  if (fabs(mu) > fabs(mu_max)) {
+    mu_max = mu;
+    ix = 0;
+  }
+  mu = (y[i1][0] * y[i2][1] + y[i0][0] * y[i1][1] + y[i2][0] * y[i0][1] - y[i1][0] * y[i0][1] -
+        y[i2][0] * y[i1][1] - y[i0][0] * y[i2][1]);
+  if (fabs(mu) > fabs(mu_max)) {
+    mu_max = mu;
+    ix = 0;
+    iy = 1;
+  }
+  double C[3];
+  bool sc[3];
+  C[0] = (p0[ix] * y[i1][iy] + p0[iy] * y[i2][ix] + y[i1][ix] * y[i2][iy] - p0[ix] * y[i2][iy] -
+          p0[iy] * y[i1][ix] - y[i2][ix] * y[i1][iy]);
+  sc[0] = svSameSign(mu_max, C[0]);
+  C[1] = (p0[ix] * y[i2][iy] + p0[iy] * y[i0][ix] + y[i2][ix] * y[i0][iy] - p0[ix] * y[i0][iy] -
+          p0[iy] * y[i2][ix] - y[i0][ix] * y[i2][iy]);
+  sc[1] = svSameSign(mu_max, C[1]);
+  C[2] = (p0[ix] * y[i0][iy] + p0[iy] * y[i1][ix] + y[i0][ix] * y[i1][iy] - p0[ix] * y[i1][iy] -
+          p0[iy] * y[i0][ix] - y[i1][ix] * y[i0][iy]);
+  sc[2] = svSameSign(mu_max, C[2]);
+  if (sc[0] && sc[1] && sc[2]) {
+    lam[i0] = C[0] / mu_max;
+    lam[i1] = C[1] / mu_max;
+    lam[i2] = C[2] / mu_max;
+    return;
+  }
+  double d = 1.e9;
+  unsigned new_bits = 0;
+  const unsigned idx[3] = {i0, i1, i2};
+  for (unsigned j = 0; j < 3; ++j) {
+    if (sc[j]) continue;
+    unsigned new_used = bits & ~(1u << idx[j]);
+    double nl[4] = {0., 0., 0., 0.};
+    svS1d(y, new_used, nl);
+    V3 np(0., 0., 0.);
+    for (unsigned i = 0; i < 4; ++i)
+      if (new_used & (1u << i)) np += nl[i] * y[i];
+    double d_star = dot(np, np);
+    if (d_star < d) {
+      new_bits = new_used;
+      d = d_star;
+      for (unsigned i = 0; i < 4; ++i) lam[i] = nl[i];
+    }
+  }
+  bits = new_bits;
+}
+
+inline void svS3d(V3 const y[4], unsigned& bits, double (&lam)[4]) {
+  double C[4];
+  C[0] = y[3][0] * y[2][1] * y[1][2] + y[2][0] * y[1][1] * y[3][2] + y[1][0] * y[3][1] * y[2][2] -
+         y[1][0] * y[2][1] * y[3][2] - y[2][0] * y[3][1] * y[1][2] - y[3][0] * y[1][1] * y[2][2];
+  C[1] = y[0][0] * y[2][1] * y[3][2] + y[2][0] * y[3][1] * y[0][2] + y[3][0] * y[0][1] * y[2][2] -
+         y[3][0] * y[2][1] * y[0][2] - y[2][0] * y[0][1] * y[3][2] - y[0][0] * y[3][1] * y[2][2];
+  C[2] = y[3][0] * y[1][1] * y[0][2] + y[1][0] * y[0][1] * y[3][2] + y[0][0] * y[3][1] * y[1][2] -
+         y[0][0] * y[1][1] * y[3][2] - y[1][0] * y[3][1] * y[0][2] - y[3][0] * y[0][1] * y[1][2];
+  C[3] = y[0][0] * y[1][1] * y[2][2] + y[1][0] * y[2][1] * y[0][2] + y[2][0] * y[0][1] * y[1][2] -
+         y[2][0] * y[1][1] * y[0][2] - y[1][0] * y[0][1] * y[2][2] - y[0][0] * y[2][1] * y[1][2];
+  double dM = C[0] + C[1] + C[2] + C[3];
+  bool sc[4];
+  for (int i = 0; i < 4; ++i) sc[i] = svSameSign(dM, C[i]);
+  if (sc[0] && sc[1] && sc[2] && sc[3]) {
+    for (int i = 0; i < 4; ++i) lam[i] = C[i] / dM;
+    return;
+  }
+  double d = 1.e9;
+  unsigned new_bits = 0;
+  for (unsigned j = 0; j < 4; ++j) {
+    if (sc[j]) continue;
+    unsigned new_used = bits & ~(1u << j);
+    double nl[4] = {0., 0., 0., 0.};
+    svS2d(y, new_used, nl);
+    V3 np(0., 0., 0.);
+    for (unsigned i = 0; i < 4; ++i)
+      if (new_used & (1u << i)) np += nl[i] * y[i];
+    double d_star = dot(np, np);
+    if (d_star < d) {
+      new_bits = new_used;
+      d = d_star;
+      for (unsigned i = 0; i < 4; ++i) lam[i] = nl[i];
+    }
+  }
+  bits = new_bits;
+}
+
+// closest_points_GJK_SV (GJK_SV.cpp:428-525)
+double gjkSVClosestPoints(ConvexRef const& a, ConvexRef const& b, Tf const& a2w, Tf const& b2w, V3& pa, V3& pb,
+                          int& nbIter) {
+  unsigned bits = 0, last = 0;
+  V3 p[4], q[4], y[4];
+  double lam[4] = {0., 0., 0., 0.};
+  int it = 0;
+  const double relError = g_col.tol, absError = 1.e-4 * relError;
+  bool acceleration = g_col.accel != 0;
+  V3 v = applyTf(a2w, support(a, V3())) - applyTf(b2w, support(b, V3()));
+  V3 w = v, d = v;
+  double dist = Norm(v);
+  while (bits < 15 && dist > EPSILON2 && it < 1000) {
+    ++it;
+    for (unsigned ni = 0; ni < 4; ++ni)
+      if (!(bits & (1u << ni))) {
+        last = ni;
+        break;
+      }
+    if (acceleration && it % 8 != 0) d = nesterovDirection(it, d, v, w);
+    else d = v;
+    p[last] = support(a, vecTimesBasis(-d, a2w));
+    q[last] = support(b, vecTimesBasis(d, b2w));
+    w = applyTf(a2w, p[last]) - applyTf(b2w, q[last]);
+    double mu = dist - dot(v, w) / dist;
+    if (mu < dist * relError || mu < absError) {
+      if (!acceleration) break;
+      if (Norm(d - v) < EPSILON) break;
+      acceleration = false;
+      p[last] = support(a, vecTimesBasis(-v, a2w));
+      q[last] = support(b, vecTimesBasis(v, b2w));
+      w = applyTf(a2w, p[last]) - applyTf(b2w, q[last]);
+    }
+    bool degen = false;
+    for (unsigned i = 0; i < 4; ++i)
+      if ((bits & (1u << i)) && y[i][0] == w[0] && y[i][1] == w[1] && y[i][2] == w[2]) degen = true;
+    if (degen) break;
+    y[last] = w;
+    bits |= 1u << last;
+    unsigned num = 0;
+    for (unsigned i = 0; i < 4; ++i) num += (bits >> i) & 1u;
+    if (num == 1) {
+      for (unsigned i = 0; i < 4; ++i)
+        if (bits & (1u << i)) lam[i] = 1.;
+    } else if (num == 2) {
+      svS1d(y, bits, lam);
+    } else if (num == 3) {
+      svS2d(y, bits, lam);
+    } else {
+      svS3d(y, bits, lam);
+    }
+    v = V3(0., 0., 0.);
+    for (unsigned i = 0; i < 4; ++i)
+      if (bits & (1u << i)) v += lam[i] * y[i];
+    dist = Norm(v);
+  }
+  pa = V3(0., 0., 0.);
+  pb = V3(0., 0., 0.);
+  for (unsigned i = 0; i < 4; ++i)
+    if (bits & (1u << i)) {
+      pa += lam[i] * p[i];
+      pb += lam[i] * q[i];
+    }
+  nbIter = it;
+  return dist;
+}
+
+// --- Bounding-volume pre-tests (OBB.cpp:144-198, OBC.cpp:1-363) ------------
+// OBB half extents of a convex at the identity (Box.cpp:1031-1056,
+// Polyhedron.cpp:776-820, Sphere.cpp:528-537): the box's, the largest |x|,
+// |y|, |z| of the polyhedron's vertices, the sphere's radius.
+inline V3 obbExtent(ConvexRef const& c) {
+  if (c.type == SH_SPHERE) return V3(c.r, c.r, c.r);
+  if (c.type == SH_BOX) return c.e;
+  V3 e(0., 0., 0.);
+  for (V3 const& pt : c.sh->verts)
+    for (int a = 0; a < 3; ++a) e[a] = fabs(pt[a]) > e[a] ? fabs(pt[a]) : e[a];
+  return e;
+}
+
+// Separating-axis test of two OBBs (OBB.cpp:144-198): the 15 axes, with the
+// 1e-6 (LOWEPS) margin on the absolute rotation
+inline bool obbOverlap(V3 const& a, Tf const& a2w, V3 const& b, Tf const& b2w) {
+  const double LOWEPS = 1.0e-6;
+  double ori[3][3], oriAbs[3][3];
+  V3 dc = b2w.o - a2w.o;
+  // ori = transpose(Ba) * Bb ; cen = transpose(Ba) * (ob - oa)
+  V3 cen;
+  for (int i = 0; i < 3; ++i) cen[i] = a2w.B[0][i] * dc[0] + a2w.B[1][i] * dc[1] + a2w.B[2][i] * dc[2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      ori[i][j] = a2w.B[0][i] * b2w.B[0][j] + a2w.B[1][i] * b2w.B[1][j] + a2w.B[2][i] * b2w.B[2][j];
+      oriAbs[i][j] = fabs(ori[i][j]) + LOWEPS;
+    }
+  for (int i = 0; i < 3; ++i)
+    if (fabs(cen[i]) > (a[i] + b[0] * oriAbs[i][0] + b[1] * oriAbs[i][1] + b[2] * oriAbs[i][2])) return false;
+  for (int i = 0; i < 3; ++i)
+    if (fabs(cen[0] * ori[0][i] + cen[1] * ori[1][i] + cen[2] * ori[2][i]) >
+        (b[i] + a[0] * oriAbs[0][i] + a[1] * oriAbs[1][i] + a[2] * oriAbs[2][i]))
+      return false;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i)
+      if (fabs(cen[(i + 2) % 3] * ori[(i + 1) % 3][j] - cen[(i + 1) % 3] * ori[(i + 2) % 3][j]) >
+          (a[(i + 1) % 3] * oriAbs[(i + 2) % 3][j] + a[(i + 2) % 3] * oriAbs[(i + 1) % 3][j] +
+           b[(j + 1) % 3] * oriAbs[i][(j + 2) % 3] + b[(j + 2) % 3] * oriAbs[i][(j + 1) % 3]))
+        return false;
+  return true;
+}
+
+// OBC of a convex (Box.cpp:1031-1056, Polyhedron.cpp:776-820,
+// Sphere.cpp:528-537): axis along the OBB extent closest to another one,
+// height twice that extent, radius the norm of the two others
+struct OBCyl {
+  double r, h;
+  V3 e;
+};
+inline OBCyl obcOf(ConvexRef const& c) {
+  if (c.type == SH_SPHERE) return OBCyl{c.r, 2. * c.r, V3(1., 0., 0.)};
+  V3 ext = obbExtent(c);
+  double xy = fabs(ext[0] - ext[1]), xz = fabs(ext[0] - ext[2]), yz = fabs(ext[1] - ext[2]);
+  int zAxis = xy < xz ? 2 : 1;
+  xy = xy < xz ? xy : xz;
+  zAxis = xy < yz ? zAxis : 0;
+  V3 e(0., 0., 0.);
+  e[zAxis] = 1.;
+  double h = 2. * ext[zAxis];
+  double r = std::sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2] - ext[zAxis] * ext[zAxis]);
+  return OBCyl{r, h, e};
+}
+
+inline int sgnOf(double v) { return (0. < v) - (v < 0.); }
+
+// Real roots of x^4 + b x^3 + c x^2 + d x + e (OBC.cpp:143-222): depressed
+// quartic, biquadratic case or Ferrari's resolvent cubic
+inline void quarticRoots(double b, double c, double d, double e, double sol[4], int& nb) {
+  nb = 0;
+  const double b2 = b * b;
+  const double p = c - 3. * b2 / 8.;
+  const double q = b2 * b / 8. - b * c / 2. + d;
+  const double r = -3. * b2 * b2 / 256. + e - b * d / 4. + b2 * c / 16.;
+  const double p2 = p * p;
+  if (fabs(q) < EPSILON) {
+    const double del = p2 / 4. - r;
+    if (del < 0.) return;
+    const double m1 = -p / 2. + std::sqrt(del), m2 = -p / 2. - std::sqrt(del);
+    if (m1 > 0.) {
+      sol[nb++] = std::sqrt(m1) - b / 4.;
+      sol[nb++] = -std::sqrt(m1) - b / 4.;
+    }
+    if (m2 > 0.) {
+      sol[nb++] = std::sqrt(m2) - b / 4.;
+      sol[nb++] = -std::sqrt(m2) - b / 4.;
+    }
+    return;
+  }
+  const double u = -p2 / 36. - r / 3.;
+  const double v = -p2 * p / 216. + r * p / 6. - q * q / 16.;
+  const double del = u * u * u + v * v;
+  double m;
+  if (del < 0) {
+    m = 2. * std::sqrt(-u) * std::cos(std::acos(v / std::sqrt(-u) / u) / 3.) - p / 3.;
+  } else {
+    m = std::cbrt(-v + std::sqrt(del));
+    m = m - u / m - p / 3.;
+  }
+  if (m < 0.) return;
+  const double sm = std::sqrt(m / 2.);
+  const double f1 = -p / 2. - m / 2. - q / sm / 4.;
+  const double f2 = f1 + q / sm / 2.;
+  if (f1 > 0.) {
+    sol[nb++] = sm + std::sqrt(f1) - b / 4.;
+    sol[nb++] = sm - std::sqrt(f1) - b / 4.;
+  }
+  if (f2 > 0.) {
+    sol[nb++] = -sm + std::sqrt(f2) - b / 4.;
+    sol[nb++] = -sm - std::sqrt(f2) - b / 4.;
+  }
+}
+
+inline V3 basisTimes(Tf const& t, V3 const& v) {
+  return V3(t.B[0][0] * v[0] + t.B[0][1] * v[1] + t.B[0][2] * v[2],
+            t.B[1][0] * v[0] + t.B[1][1] * v[1] + t.B[1][2] * v[2],
+            t.B[2][0] * v[0] + t.B[2][1] * v[1] + t.B[2][2] * v[2]);
+}
+inline V3 unitOf(V3 const& v) { return v / Norm(v); }
+
+// One ellipse-circle step of the OBC test (OBC.cpp:300-355): the other
+// cylinder's cap seen as an ellipse against this one's circle of radius rc
+inline bool obcEllipseApart(double x0, double yTop, double yBottom, double ez, double ey, double re, double rc) {
+  if (std::signbit(yTop) != std::signbit(yBottom)) return false;
+  const double y0 = fabs(yTop) > fabs(yBottom) ? yBottom : yTop;
+  if (!(x0 * x0 + y0 * y0 / ez / ez > re * re)) return false;
+  const double ey_sq = ey * ey;
+  const double A = re * re * ey_sq * ey_sq;
+  const double B = 2. * y0 * ez / re / ey_sq;
+  const double C = x0 * x0 / A;
+  const double D = y0 * y0 * ez * ez / A;
+  double sint[4];
+  int nb;
+  quarticRoots(-B, C + D - 1., B, -D, sint, nb);
+  for (int i = 0; i < nb; ++i)
+    if (std::signbit(sint[i]) != std::signbit(B)) {
+      const double cost = -sgnOf(x0) * std::sqrt(1. - sint[i] * sint[i]);
+      const double ptX = x0 + re * cost;
+      const double ptY = y0 + re * ez * sint[i];
+      if (ptX * ptX + ptY * ptY > rc * rc) return true;
+    }
+  return false;
+}
+
+// isContactBVolume(OBC, OBC) (OBC.cpp:226-363)
+inline bool obcOverlap(OBCyl const& A, Tf const& a2w, OBCyl const& Bc, Tf const& b2w) {
+  const double r1 = A.r, r2 = Bc.r, h1 = A.h / 2., h2 = Bc.h / 2.;
+  const V3 eZ = basisTimes(a2w, A.e);
+  const V3 e2 = basisTimes(b2w, Bc.e);
+  const V3 xPt = b2w.o - a2w.o;
+  const V3 eX = unitOf(cross(e2, eZ));
+  const V3 eY = unitOf(cross(eZ, eX));
+  const double x = dot(eX, xPt), y = dot(eY, xPt), z = dot(eZ, xPt);
+  const double ey = dot(eY, e2), ez = dot(eZ, e2);
+  if (fabs(x) >= r1 + r2) return false;
+  {
+    const double s2 = y / ey;
+    const double s1 = z + s2 * ez;
+    if (fabs(s1) < h1 && fabs(s2) < h2) return true;
+  }
+  {
+    const double fey = fabs(ey) + 1.e-5, fez = fabs(ez) + 1.e-5;
+    if (fabs(y) > r1 + h2 * fey + r2 * fez) return false;
+    if (fabs(z) > h1 + h2 * fez + r2 * fey) return false;
+    if (fabs(y * ez - z * ey) > r2 + h1 * fey + r1 * fez) return false;
+    if (fabs(y * ey + z * ez) > h2 + h1 * fez + r1 * fey) return false;
+  }
+  if (fabs(ez) < 0.99 && fabs(ez) > 0.01) {
+    if (obcEllipseApart(x, y + h2 * ey, y - h2 * ey, ez, ey, r2, r1)) return false;
+    const V3 eY2 = unitOf(cross(eX, e2));
+    const double y2 = -dot(eY2, xPt);
+    const double ey2 = dot(eY2, eZ);
+    if (obcEllipseApart(-x, y2 + h1 * ey2, y2 - h1 * ey2, ez, ey2, r1, r2)) return false;
+  }
+  return true;
+}
+
+// RigidBodyWithCrust::isContactBVolume (RigidBodyWithCrust.cpp:1095-1100)
+// with the full transforms; bv = 1 OBB, 2 OBC
+inline bool bvOverlap(int bv, ConvexRef const& a, Tf const& a2w, ConvexRef const& b, Tf const& b2w) {
+  if (bv == 1) return obbOverlap(obbExtent(a), a2w, obbExtent(b), b2w);
+  return obcOverlap(obcOf(a), a2w, obcOf(b), b2w);
 }
 
 // --- Shapes: construction ---------------------------------------------------

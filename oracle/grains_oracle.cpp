@@ -234,6 +234,7 @@ struct System {
   std::vector<int> insertion;            // insertion order of ids
   std::vector<Obs> obstacles;
   std::vector<Shape> shapes;
+  CollisionOptions col;                  // <CollisionDetectionAlgorithm>
   bool initialized;
   // diagnostics
   std::vector<PairRec> pairs;
@@ -693,10 +694,18 @@ bool closestPointGJK(Geo const& A, Geo const& B, Contact& c) {
     c.d = 1.e20;
     return true;
   }
+  // isContactBVolume on the full transforms (RigidBodyWithCrust.cpp:283-284)
+  if (g_col.bv && !bvOverlap(g_col.bv, A.c, A.tf, B.c, B.tf)) {
+    c.any = false;
+    c.d = 1.e20;
+    return true;
+  }
   Tf a2w = scaledTf(A, A.scaling), b2w = scaledTf(B, B.scaling);
   V3 pa, pb;
   int nb = 0;
-  double distance = gjkClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb);
+  // GJK_SV or Johnson as the input chose (RigidBodyWithCrust.cpp:304-310)
+  double distance = g_col.sv ? gjkSVClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb)
+                             : gjkClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb);
   ++g_gjk_calls;
   g_gjk_iters += nb;
   c.nit = nb;
@@ -1657,6 +1666,7 @@ int orc_shape_info(void* h, int shape, double* out) {
 double orc_gjk(void* h, int shapeA, const double* BA, const double* oA, int shapeB, const double* BB,
                const double* oB, int crust, double* pa, double* pb, int* iters) {
   System* s = (System*)h;
+  g_col = s->col;
   Shape const& sa = s->shapes[shapeA];
   Shape const& sb = s->shapes[shapeB];
   int ca = sa.start_vertex, cb = sb.start_vertex;
@@ -1690,6 +1700,7 @@ double orc_gjk(void* h, int shapeA, const double* BA, const double* oA, int shap
 int orc_closest_point(void* h, int shapeA, const double* xA, const double* BA, int shapeB, const double* xB,
                       const double* BB, double* out) {
   System* s = (System*)h;
+  g_col = s->col;
   auto mk = [&](int sh, const double* x, const double* B) {
     Shape const& d = s->shapes[sh];
     Part p;
@@ -1810,8 +1821,41 @@ int orc_add_box(void* h, const double* L, const double* centre, const double* ba
 
 // Builds the grid, links particles and obstacles, creates periodic clones
 // and primes the LeapFrog forces (Grains.cpp:95-179, 1010-1033, 1965-2010).
+// The bounding-volume pre-test (bv 1 OBB, 2 OBC) of two shape classes
+// placed at origins o with bases B (rows); 1 when the volumes meet
+int orc_bv_overlap(void* h, int bv, int shapeA, const double* BA, const double* oA, int shapeB, const double* BB,
+                   const double* oB) {
+  System* s = (System*)h;
+  Shape const& sa = s->shapes[shapeA];
+  Shape const& sb = s->shapes[shapeB];
+  int ca = 0, cb = 0;
+  ConvexRef A{sa.type, sa.radius, sa.extent, &sa, &ca}, B{sb.type, sb.radius, sb.extent, &sb, &cb};
+  Tf ta, tb;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      ta.B[i][j] = BA[3 * i + j];
+      tb.B[i][j] = BB[3 * i + j];
+    }
+  ta.o = V3(oA[0], oA[1], oA[2]);
+  tb.o = V3(oB[0], oB[1], oB[2]);
+  return bvOverlap(bv, A, ta, B, tb) ? 1 : 0;
+}
+
+// <CollisionDetectionAlgorithm>: sv 1 = GJK_SV, tol = Tolerance,
+// accel 1 = Acceleration ON, bv = BoundingVolume (0 none, 1 OBB, 2 OBC)
+int orc_set_collision(void* h, int sv, double tol, int accel, int bv) {
+  System* s = (System*)h;
+  if (tol < 1e-10 || bv < 0 || bv > 2) return 1;
+  s->col.sv = sv ? 1 : 0;
+  s->col.tol = tol;
+  s->col.accel = accel ? 1 : 0;
+  s->col.bv = bv;
+  return 0;
+}
+
 int orc_init(void* h) {
   System* s = (System*)h;
+  g_col = s->col;
   double rmax = 0.;
   for (auto& p : s->pool) rmax = std::max(rmax, p.radius);
   if (rmax < 1.e-12) rmax = 1.e16;
@@ -1867,6 +1911,7 @@ int orc_init(void* h) {
 // MotionError 2, SimulationError 3)
 int orc_step(void* h, int n) {
   System* s = (System*)h;
+  g_col = s->col;
   for (int i = 0; i < n; ++i) {
     int e = oneStep(s);
     if (e) return s->err = e;
@@ -1982,6 +2027,7 @@ void orc_get_stats(void* h, double* out) {
 // are rebuilt.  Forces are recomputed if recompute != 0.
 int orc_set_state(void* h, const double* x, const double* v, const double* w, const double* q, int recompute) {
   System* s = (System*)h;
+  g_col = s->col;
   int n = (int)s->insertion.size();
   // drop clones and all cell links
   for (auto& kv : s->clones) s->pool[kv.second].alive = false;

@@ -37,6 +37,8 @@ def lib():
         L.orc_add_spheres.argtypes = [vp, ctypes.c_int, dp, dp, dp, ip, dp, dp, dp]
         L.orc_add_box.argtypes = [vp, dp, dp, dp, ctypes.c_double, ctypes.c_int]
         L.orc_init.argtypes = [vp]
+        L.orc_set_collision.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_bv_overlap.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp]
         L.orc_step.argtypes = [vp, ctypes.c_int]
         L.orc_num_particles.argtypes = [vp]
         L.orc_num_active.argtypes = [vp]
@@ -108,6 +110,19 @@ class Oracle:
 
     def contact_model(self, matA, matB, kn, en, muc, kr=0.0, etat=-1.0):
         lib().orc_set_contact_model(self.h, matA, matB, kn, en, etat, muc, kr)
+
+    def collision(self, gjk_sv=False, tolerance=1e-10, acceleration=False, bounding_volume=0):
+        """<CollisionDetectionAlgorithm>: GJK_SV or Johnson, relative
+        tolerance, Nesterov acceleration, pre-test 0 none / 1 OBB / 2 OBC."""
+        if lib().orc_set_collision(self.h, int(bool(gjk_sv)), float(tolerance), int(bool(acceleration)),
+                                   int(bounding_volume)):
+            raise ValueError("bad collision options")
+
+    def bv_overlap(self, bv, shapeA, BA, oA, shapeB, BB, oB):
+        """Bounding-volume pre-test (1 OBB, 2 OBC) of two placed shapes."""
+        a = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1)) for x in (BA, oA, BB, oB)]
+        return bool(lib().orc_bv_overlap(self.h, int(bv), int(shapeA), _d(a[0]), _d(a[1]), int(shapeB), _d(a[2]),
+                                         _d(a[3])))
 
     def memory_contact_model(self, matA, matB, kn, en, muc, kt, etat=-1.0, mur=0.0, etarpf=0.0, epst=0.0):
         """HookeMemory (tangential spring kt, rolling friction mur)."""
