@@ -127,6 +127,16 @@ int grains_gpu_add_particles(grains_gpu_system* sys, int shape, size_t n, const 
 /* out = {volume, circumscribed radius, inertia xx xy xz yy yz zz, crust
  * scaling[3], bounding-box half extents[3]}, per unit density */
 int grains_gpu_shape_info(grains_gpu_system* sys, int shape, double* out);
+/* <CollisionDetectionAlgorithm> of Grains::Construction (Grains.cpp:640-770;
+ * GrainsExec::m_colDetGJK_SV, m_colDetTolerance, m_colDetAcceleration,
+ * m_colDetBoundingVolume), before initialisation: gjk_sv 1 = the
+ * signed-volume sub-distance (GJK_SV.cpp:428-525) for the general pairs,
+ * 0 = Johnson's (the ContactError recovery uses Johnson's either way);
+ * tolerance = relative tolerance of the optimality gap (>= 1e-10, default
+ * 1e-10); acceleration 1 = Nesterov; bounding_volume 1 = OBB, 2 = OBC
+ * pre-test of general pairs (OBB.cpp:144, OBC.cpp:226), 0 = none. */
+int grains_gpu_set_collision(grains_gpu_system* sys, int gjk_sv, double tolerance, int acceleration,
+                             int bounding_volume);
 /* The device narrow phase evaluated on the host for two bodies of shape
  * classes shapeA, shapeB at centres x and row-major bases B
  * (RigidBodyWithCrust::ClosestPoint and its ContactError handling,

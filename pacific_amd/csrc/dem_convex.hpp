@@ -37,6 +37,8 @@ struct ConvexGeo {
   d3 scaling;     // crust shrink factors
   d3 bext;        // bbox half extents at the identity transform
   double crust, circR;
+  d3 obb;         // OBB half extents at the identity (bounding-volume pre-test)
+  GjkOpts opt;    // the system's narrow-phase options
 };
 
 struct Tf {
@@ -304,18 +306,44 @@ DEM_HD bool j_degenerate(const Simplex& S, unsigned all_bits, d3 w) {
 // than sep (the lower bound v.w / |v| of the distance exceeds it, with a
 // margin above the rounding of v.w), so that a pair that cannot be in
 // contact stops at once; the distance itself is then not computed.
+// Nesterov direction of closest_points (Convex.cpp:575-585; GJK_SV.cpp:479-489)
+DEM_HD d3 nesterov_dir(int it, d3 d, d3 v, d3 w) {
+  const double momentum = it / (it + 2.), omm = 1. - momentum;
+  return momentum * d + momentum * omm * v + omm * omm * w;
+}
+
+// The search direction, support points and optimality-gap test of one
+// iteration, with the tolerance and acceleration of o: 1 to go on with the
+// support points P, Q (local frames) and w, 0 when the loop ends, 2 when w
+// proves the distance larger than sep (sep >= 0, acceleration off).
+DEM_HD int gjk_search(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, const GjkOpts& o,
+                      int it, d3 v, double dist, double sep, d3& d, d3& w, bool& accel, d3& P, d3& Q) {
+  const double relError = o.tol, absError = 1.e-4 * relError;
+  d = (accel && it % 8 != 0) ? nesterov_dir(it, d, v, w) : v;
+  P = support(A, tf_vec_basis(-d, a2w));
+  Q = support(B, tf_vec_basis(d, b2w));
+  w = tf_apply(a2w, P) - tf_apply(b2w, Q);
+  if (sep >= 0. && !o.accel && dot(v, w) > sep * dist + 1.e-12 * dist * (fabs(w.x) + fabs(w.y) + fabs(w.z)))
+    return 2;
+  const double mu = dist - dot(v, w) / dist;
+  if (mu < dist * relError || mu < absError) {
+    if (!accel) return 0;
+    if (norm(d - v) < kEps) return 0;
+    accel = false;
+    P = support(A, tf_vec_basis(-v, a2w));
+    Q = support(B, tf_vec_basis(v, b2w));
+    w = tf_apply(a2w, P) - tf_apply(b2w, Q);
+  }
+  return 1;
+}
+
 template <int LAST>
 DEM_HD int gjk_iter(Simplex& S, const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w,
-                    unsigned& bits, unsigned& all_bits, d3& v, double& dist, double sep) {
+                    unsigned& bits, unsigned& all_bits, d3& v, double& dist, double sep, int it, d3& d, d3& w,
+                    bool& accel) {
   constexpr unsigned last_bit = 1u << LAST;
-  const double relError = kEps, absError = 1.e-4 * relError;
-  d3 d = v;
-  S.p[LAST] = support(A, tf_vec_basis(-d, a2w));
-  S.q[LAST] = support(B, tf_vec_basis(d, b2w));
-  d3 w = tf_apply(a2w, S.p[LAST]) - tf_apply(b2w, S.q[LAST]);
-  if (sep >= 0. && dot(v, w) > sep * dist + 1.e-12 * dist * (fabs(w.x) + fabs(w.y) + fabs(w.z))) return 2;
-  double mu = dist - dot(v, w) / dist;
-  if (mu < dist * relError || mu < absError) return 0;
+  const int go = gjk_search(A, a2w, B, b2w, A.opt, it, v, dist, sep, d, w, accel, S.p[LAST], S.q[LAST]);
+  if (go != 1) return go;
   if (j_degenerate(S, all_bits, w)) return 0;
   S.y[LAST] = w;
   all_bits = bits | last_bit;
@@ -324,12 +352,454 @@ DEM_HD int gjk_iter(Simplex& S, const ConvexGeo& A, const Tf& a2w, const ConvexG
   return 1;
 }
 
-// closest_points with relative tolerance 1e-10 (Convex.cpp:536-627);
-// witness points in the local frames; returns the distance
-// (sep >= 0: *separated set when the distance is proven larger than sep;
-// the witness points are then not computed)
-DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, d3& pa, d3& pb,
-                          int& iters, double sep = -1., bool* separated = nullptr) {
+// --- Signed-volume sub-distance (GJK_SV.cpp:1-420) --------------------------
+// Montanari, Petrinic and Barbieri (2017): the closest point of the simplex
+// from signed volumes of its sub-simplices in the projection of largest
+// area, recursing into the faces (edges) whose volume disagrees in sign.
+// The points are named by their slot bits as in Johnson's; every slot index
+// is a template argument so that the simplex lives in registers.
+// Barycentric weights of the four slots, as named scalars (an array of
+// them, written through references, would sit in scratch memory).
+struct L4 {
+  double w0, w1, w2, w3;
+};
+template <int I>
+DEM_HD double& lw(L4& l) {
+  if constexpr (I == 0) return l.w0;
+  else if constexpr (I == 1) return l.w1;
+  else if constexpr (I == 2) return l.w2;
+  else return l.w3;
+}
+
+struct SvSimplex {
+  d3 y[4], p[4], q[4];
+  L4 lam;
+};
+
+DEM_HD bool sv_same_sign(double a, double b) { return !((a > 0) ^ (b > 0)); }
+// component C of v, C a compile-time constant
+template <int C>
+DEM_HD double cget(const d3& v) {
+  return C == 0 ? v.x : (C == 1 ? v.y : v.z);
+}
+
+// sum of lam_i y_i over the slots of bits, in slot order
+DEM_HD d3 sv_point(const d3 (&y)[4], unsigned bits, const L4& l) {
+  d3 v{0., 0., 0.};
+  if (bits & 1u) v += l.w0 * y[0];
+  if (bits & 2u) v += l.w1 * y[1];
+  if (bits & 4u) v += l.w2 * y[2];
+  if (bits & 8u) v += l.w3 * y[3];
+  return v;
+}
+
+template <int I1, int I2, int I>
+DEM_HD void sv_s1d_tail(const d3 (&y)[4], const d3& t, double neg_tI, unsigned& bits, L4& lam) {
+  const double pI = dot(y[I2], t) / dot(t, t) * neg_tI + cget<I>(y[I2]);
+  const double C0 = -cget<I>(y[I2]) + pI, C1 = cget<I>(y[I1]) - pI;
+  const bool s0 = sv_same_sign(neg_tI, C0), s1 = sv_same_sign(neg_tI, C1);
+  // both weights written in every case (values selected, not addresses:
+  // a store through a selected address would put the weights in scratch)
+  const bool both = s0 && s1;
+  const double l1 = both ? C0 / neg_tI : (s0 ? 1. : lw<I1>(lam));
+  const double l2 = both ? C1 / neg_tI : (s0 ? lw<I2>(lam) : 1.);
+  lw<I1>(lam) = l1;
+  lw<I2>(lam) = l2;
+  if (!both) bits &= ~(1u << (s0 ? I2 : I1));
+}
+
+// the point of the edge I1-I2 closest to the origin (GJK_SV.cpp:19-93)
+template <int I1, int I2>
+DEM_HD void sv_s1d(const d3 (&y)[4], unsigned& bits, L4& lam) {
+  const d3 t = y[I2] - y[I1];
+  int I = 0;
+  double neg_tI = -t.x;
+  if (fabs(t.y) > fabs(neg_tI)) {
+    I = 1;
+    neg_tI = -t.y;
+  }
+  if (fabs(t.z) > fabs(neg_tI)) {
+    I = 2;
+    neg_tI = -t.z;
+  }
+  if (I == 0) sv_s1d_tail<I1, I2, 0>(y, t, neg_tI, bits, lam);
+  else if (I == 1) sv_s1d_tail<I1, I2, 1>(y, t, neg_tI, bits, lam);
+  else sv_s1d_tail<I1, I2, 2>(y, t, neg_tI, bits, lam);
+}
+
+template <int I0, int I1, int I2>
+DEM_HD void sv_s2d(const d3 (&y)[4], unsigned& bits, L4& lam);
+
+// The recursion into the sub-simplex without slot J (edges of a triangle,
+// faces of a tetrahedron) keeps it when its point is the closest so far
+// (GJK_SV.cpp:190-225, 338-370)
+template <int J, int A0, int A1, int A2 = -1>
+DEM_HD void sv_keep(const d3 (&y)[4], unsigned bits, double& dbest, unsigned& best_bits, L4& lam) {
+  unsigned used = bits & ~(1u << J);
+  L4 nl{0., 0., 0., 0.};
+  if constexpr (A2 < 0) sv_s1d<A0, A1>(y, used, nl);
+  else sv_s2d<A0, A1, A2>(y, used, nl);
+  const d3 np = sv_point(y, used, nl);
+  const double ds = dot(np, np);
+  if (ds < dbest) {
+    best_bits = used;
+    dbest = ds;
+    lam = nl;
+  }
+}
+
+// signed areas of the sub-triangles with the projected origin p0 in the
+// plane of axes X, Y (GJK_SV.cpp:161-188)
+template <int X, int Y>
+DEM_HD void sv_areas(d3 p0, d3 a, d3 b, d3 c, double& C0, double& C1, double& C2) {
+  const double px = cget<X>(p0), py = cget<Y>(p0);
+  const double y0x = cget<X>(a), y0y = cget<Y>(a), y1x = cget<X>(b), y1y = cget<Y>(b), y2x = cget<X>(c),
+               y2y = cget<Y>(c);
+  C0 = (px * y1y + py * y2x + y1x * y2y - px * y2y - py * y1x - y2x * y1y);
+  C1 = (px * y2y + py * y0x + y2x * y0y - px * y0y - py * y2x - y0x * y2y);
+  C2 = (px * y0y + py * y1x + y0x * y1y - px * y1y - py * y0x - y1x * y0y);
+}
+
+// the triangle I0-I1-I2 (GJK_SV.cpp:99-228)
+template <int I0, int I1, int I2>
+DEM_HD void sv_s2d(const d3 (&y)[4], unsigned& bits, L4& lam) {
+  const d3 n = cross(y[I1] - y[I0], y[I2] - y[I0]);
+  const d3 p0 = (dot(y[I0], n) / dot(n, n)) * n;
+  int ix = 1, iy = 2;
+  double mu_max = (y[I1].y * y[I2].z + y[I0].y * y[I1].z + y[I2].y * y[I0].z - y[I1].y * y[I0].z -
+                   y[I2].y * y[I1].z - y[I0].y * y[I2].z);
+  double mu = (y[I1].z * y[I0].x + y[I2].z * y[I1].x + y[I0].z * y[I2].x - y[I1].z * y[I2].x -
+               y[I0].z * y[I1].x - y[I2].z * y[I0].x);
+  if (fabs(mu) > fabs(mu_max)) {
+    mu_max = mu;
+    ix = 0;
+  }
+  mu = (y[I1].x * y[I2].y + y[I0].x * y[I1].y + y[I2].x * y[I0].y - y[I1].x * y[I0].y - y[I2].x * y[I1].y -
+        y[I0].x * y[I2].y);
+  if (fabs(mu) > fabs(mu_max)) {
+    mu_max = mu;
+    ix = 0;
+    iy = 1;
+  }
+  double C0, C1, C2;
+  if (ix == 1) sv_areas<1, 2>(p0, y[I0], y[I1], y[I2], C0, C1, C2);
+  else if (iy == 2) sv_areas<0, 2>(p0, y[I0], y[I1], y[I2], C0, C1, C2);
+  else sv_areas<0, 1>(p0, y[I0], y[I1], y[I2], C0, C1, C2);
+  const bool s0 = sv_same_sign(mu_max, C0), s1 = sv_same_sign(mu_max, C1), s2 = sv_same_sign(mu_max, C2);
+  if (s0 && s1 && s2) {
+    lw<I0>(lam) = C0 / mu_max;
+    lw<I1>(lam) = C1 / mu_max;
+    lw<I2>(lam) = C2 / mu_max;
+    return;
+  }
+  double dbest = 1.e9;
+  unsigned best_bits = 0;
+  if (!s0) sv_keep<I0, I1, I2>(y, bits, dbest, best_bits, lam);
+  if (!s1) sv_keep<I1, I0, I2>(y, bits, dbest, best_bits, lam);
+  if (!s2) sv_keep<I2, I0, I1>(y, bits, dbest, best_bits, lam);
+  bits = best_bits;
+}
+
+// the tetrahedron (GJK_SV.cpp:234-373)
+DEM_HD void sv_s3d(const d3 (&y)[4], unsigned& bits, L4& lam) {
+  const double C0 = y[3].x * y[2].y * y[1].z + y[2].x * y[1].y * y[3].z + y[1].x * y[3].y * y[2].z -
+                    y[1].x * y[2].y * y[3].z - y[2].x * y[3].y * y[1].z - y[3].x * y[1].y * y[2].z;
+  const double C1 = y[0].x * y[2].y * y[3].z + y[2].x * y[3].y * y[0].z + y[3].x * y[0].y * y[2].z -
+                    y[3].x * y[2].y * y[0].z - y[2].x * y[0].y * y[3].z - y[0].x * y[3].y * y[2].z;
+  const double C2 = y[3].x * y[1].y * y[0].z + y[1].x * y[0].y * y[3].z + y[0].x * y[3].y * y[1].z -
+                    y[0].x * y[1].y * y[3].z - y[1].x * y[3].y * y[0].z - y[3].x * y[0].y * y[1].z;
+  const double C3 = y[0].x * y[1].y * y[2].z + y[1].x * y[2].y * y[0].z + y[2].x * y[0].y * y[1].z -
+                    y[2].x * y[1].y * y[0].z - y[1].x * y[0].y * y[2].z - y[0].x * y[2].y * y[1].z;
+  const double dM = C0 + C1 + C2 + C3;
+  const bool s0 = sv_same_sign(dM, C0), s1 = sv_same_sign(dM, C1), s2 = sv_same_sign(dM, C2),
+             s3 = sv_same_sign(dM, C3);
+  if (s0 && s1 && s2 && s3) {
+    lam.w0 = C0 / dM;
+    lam.w1 = C1 / dM;
+    lam.w2 = C2 / dM;
+    lam.w3 = C3 / dM;
+    return;
+  }
+  double dbest = 1.e9;
+  unsigned best_bits = 0;
+  if (!s0) sv_keep<0, 1, 2, 3>(y, bits, dbest, best_bits, lam);
+  if (!s1) sv_keep<1, 0, 2, 3>(y, bits, dbest, best_bits, lam);
+  if (!s2) sv_keep<2, 0, 1, 3>(y, bits, dbest, best_bits, lam);
+  if (!s3) sv_keep<3, 0, 1, 2>(y, bits, dbest, best_bits, lam);
+  bits = best_bits;
+}
+
+// sv_subalgorithm (GJK_SV.cpp:380-410): by the number of points of the
+// simplex, each pattern of slots with its compile-time indices
+DEM_HD void sv_sub(const d3 (&y)[4], unsigned& bits, L4& lam) {
+  switch (bits) {
+    case 1u: lam.w0 = 1.; break;
+    case 2u: lam.w1 = 1.; break;
+    case 4u: lam.w2 = 1.; break;
+    case 8u: lam.w3 = 1.; break;
+    case 3u: sv_s1d<0, 1>(y, bits, lam); break;
+    case 5u: sv_s1d<0, 2>(y, bits, lam); break;
+    case 6u: sv_s1d<1, 2>(y, bits, lam); break;
+    case 9u: sv_s1d<0, 3>(y, bits, lam); break;
+    case 10u: sv_s1d<1, 3>(y, bits, lam); break;
+    case 12u: sv_s1d<2, 3>(y, bits, lam); break;
+    case 7u: sv_s2d<0, 1, 2>(y, bits, lam); break;
+    case 11u: sv_s2d<0, 1, 3>(y, bits, lam); break;
+    case 13u: sv_s2d<0, 2, 3>(y, bits, lam); break;
+    case 14u: sv_s2d<1, 2, 3>(y, bits, lam); break;
+    default: sv_s3d(y, bits, lam); break;
+  }
+}
+
+// One iteration of closest_points_GJK_SV with the new point in slot LAST
+// (a compile-time index, as in gjk_iter): 1 to go on, 0 when the loop ends,
+// 2 when the pair is proven farther apart than sep.
+template <int LAST>
+DEM_HD int sv_iter(SvSimplex& S, const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w,
+                   unsigned& bits, d3& v, double& dist, double sep, int it, d3& d, d3& w, bool& accel) {
+  const int go = gjk_search(A, a2w, B, b2w, A.opt, it, v, dist, sep, d, w, accel, S.p[LAST], S.q[LAST]);
+  if (go != 1) return go;
+  bool degen = false;
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if ((bits & (1u << i)) && S.y[i].x == w.x && S.y[i].y == w.y && S.y[i].z == w.z) degen = true;
+  });
+  if (degen) return 0;
+  S.y[LAST] = w;
+  bits |= 1u << LAST;
+  sv_sub(S.y, bits, S.lam);
+  v = sv_point(S.y, bits, S.lam);
+  dist = norm(v);
+  return 1;
+}
+
+// closest_points_GJK_SV (GJK_SV.cpp:428-525), tolerance and acceleration of
+// A.opt; witness points in the local frames; returns the distance
+DEM_HD double gjk_sv_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, d3& pa, d3& pb,
+                             int& iters, double sep, bool* separated) {
+  SvSimplex S;
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    S.y[i] = S.p[i] = S.q[i] = d3{0., 0., 0.};
+  });
+  S.lam = L4{0., 0., 0., 0.};
+  unsigned bits = 0;
+  const d3 zero{0., 0., 0.};
+  d3 v = tf_apply(a2w, support(A, zero)) - tf_apply(b2w, support(B, zero));
+  d3 w = v, d = v;
+  bool accel = A.opt.accel != 0;
+  double dist = norm(v);
+  int it = 0;
+  while (bits < 15 && dist > kEps2 && it < 1000) {
+    ++it;
+    int go;
+    if (!(bits & 1u)) go = sv_iter<0>(S, A, a2w, B, b2w, bits, v, dist, sep, it, d, w, accel);
+    else if (!(bits & 2u)) go = sv_iter<1>(S, A, a2w, B, b2w, bits, v, dist, sep, it, d, w, accel);
+    else if (!(bits & 4u)) go = sv_iter<2>(S, A, a2w, B, b2w, bits, v, dist, sep, it, d, w, accel);
+    else go = sv_iter<3>(S, A, a2w, B, b2w, bits, v, dist, sep, it, d, w, accel);
+    if (go == 2) {
+      if (separated) *separated = true;
+      iters = it;
+      return dist;
+    }
+    if (!go) break;
+  }
+  pa = d3{0., 0., 0.};
+  pb = d3{0., 0., 0.};
+  sfor<0, 4>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if (bits & (1u << i)) {
+      pa += lw<i>(S.lam) * S.p[i];
+      pb += lw<i>(S.lam) * S.q[i];
+    }
+  });
+  iters = it;
+  return dist;
+}
+
+// --- Bounding-volume pre-tests (OBB.cpp:144-198, OBC.cpp:226-363) -----------
+// On the full transforms (RigidBodyWithCrust::isContactBVolume,
+// RigidBodyWithCrust.cpp:1095-1100).
+
+// separating-axis test of two OBBs (Gottschalk et al. 1996): the face
+// normals of both and their nine cross products, with a 1e-6 margin on the
+// absolute rotation
+DEM_HD bool obb_overlap(const ConvexGeo& A, const ConvexGeo& B) {
+  constexpr double kLowEps = 1.0e-6;
+  const double a[3] = {A.obb.x, A.obb.y, A.obb.z}, b[3] = {B.obb.x, B.obb.y, B.obb.z};
+  const d3 dc = B.o - A.o;
+  double cen[3], ori[3][3], oa[3][3];
+  for (int i = 0; i < 3; ++i) cen[i] = A.B[i] * dc.x + A.B[3 + i] * dc.y + A.B[6 + i] * dc.z;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      ori[i][j] = A.B[i] * B.B[j] + A.B[3 + i] * B.B[3 + j] + A.B[6 + i] * B.B[6 + j];
+      oa[i][j] = fabs(ori[i][j]) + kLowEps;
+    }
+  for (int i = 0; i < 3; ++i)
+    if (fabs(cen[i]) > (a[i] + b[0] * oa[i][0] + b[1] * oa[i][1] + b[2] * oa[i][2])) return false;
+  for (int i = 0; i < 3; ++i)
+    if (fabs(cen[0] * ori[0][i] + cen[1] * ori[1][i] + cen[2] * ori[2][i]) >
+        (b[i] + a[0] * oa[0][i] + a[1] * oa[1][i] + a[2] * oa[2][i]))
+      return false;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      if (fabs(cen[i2] * ori[i1][j] - cen[i1] * ori[i2][j]) >
+          (a[i1] * oa[i2][j] + a[i2] * oa[i1][j] + b[j1] * oa[i][j2] + b[j2] * oa[i][j1]))
+        return false;
+    }
+  return true;
+}
+
+// OBC of a body from its OBB extents (Box/Polyhedron::computeBVolume): the
+// axis along the extent closest to another one, height twice that extent,
+// radius the norm of the two others; spheres: radius r, height 2r, axis x
+DEM_HD void obc_of(const ConvexGeo& g, double& r, double& h, d3& e) {
+  if (g.type == kShapeSphere) {
+    r = g.r;
+    h = 2. * g.r;
+    e = d3{1., 0., 0.};
+    return;
+  }
+  const double ex = g.obb.x, ey = g.obb.y, ez = g.obb.z;
+  double xy = fabs(ex - ey);
+  const double xz = fabs(ex - ez), yz = fabs(ey - ez);
+  int z = xy < xz ? 2 : 1;
+  xy = xy < xz ? xy : xz;
+  z = xy < yz ? z : 0;
+  const double ez2 = z == 0 ? ex : (z == 1 ? ey : ez);
+  e = d3{z == 0 ? 1. : 0., z == 1 ? 1. : 0., z == 2 ? 1. : 0.};
+  h = 2. * ez2;
+  r = sqrt(ex * ex + ey * ey + ez * ez - ez2 * ez2);
+}
+
+// real roots of x^4 + b x^3 + c x^2 + d x + e: depressed quartic, the
+// biquadratic case or Ferrari's resolvent cubic (OBC.cpp:143-222)
+DEM_HD int quartic_roots(double b, double c, double d, double e, double (&sol)[4]) {
+  int nb = 0;
+  const double b2 = b * b;
+  const double p = c - 3. * b2 / 8.;
+  const double q = b2 * b / 8. - b * c / 2. + d;
+  const double r = -3. * b2 * b2 / 256. + e - b * d / 4. + b2 * c / 16.;
+  const double p2 = p * p;
+  if (fabs(q) < kEps) {
+    const double del = p2 / 4. - r;
+    if (del < 0.) return 0;
+    const double m1 = -p / 2. + sqrt(del), m2 = -p / 2. - sqrt(del);
+    if (m1 > 0.) {
+      sol[nb++] = sqrt(m1) - b / 4.;
+      sol[nb++] = -sqrt(m1) - b / 4.;
+    }
+    if (m2 > 0.) {
+      sol[nb++] = sqrt(m2) - b / 4.;
+      sol[nb++] = -sqrt(m2) - b / 4.;
+    }
+    return nb;
+  }
+  const double u = -p2 / 36. - r / 3.;
+  const double v = -p2 * p / 216. + r * p / 6. - q * q / 16.;
+  const double del = u * u * u + v * v;
+  double m;
+  if (del < 0) {
+    m = 2. * sqrt(-u) * cos(acos(v / sqrt(-u) / u) / 3.) - p / 3.;
+  } else {
+    m = cbrt(-v + sqrt(del));
+    m = m - u / m - p / 3.;
+  }
+  if (m < 0.) return 0;
+  const double sm = sqrt(m / 2.);
+  const double f1 = -p / 2. - m / 2. - q / sm / 4.;
+  const double f2 = f1 + q / sm / 2.;
+  if (f1 > 0.) {
+    sol[nb++] = sm + sqrt(f1) - b / 4.;
+    sol[nb++] = sm - sqrt(f1) - b / 4.;
+  }
+  if (f2 > 0.) {
+    sol[nb++] = -sm + sqrt(f2) - b / 4.;
+    sol[nb++] = -sm - sqrt(f2) - b / 4.;
+  }
+  return nb;
+}
+
+// one ellipse step of the OBC test: the other cylinder's cap as an ellipse
+// (semi-axis re) against this one's circle of radius rc; true when a point
+// of the ellipse boundary lies outside the circle
+DEM_HD bool obc_ellipse_apart(double x0, double yTop, double yBottom, double ez, double ey, double re, double rc) {
+  if (__builtin_signbit(yTop) != __builtin_signbit(yBottom)) return false;
+  const double y0 = fabs(yTop) > fabs(yBottom) ? yBottom : yTop;
+  if (!(x0 * x0 + y0 * y0 / ez / ez > re * re)) return false;
+  const double ey_sq = ey * ey;
+  const double A = re * re * ey_sq * ey_sq;
+  const double B = 2. * y0 * ez / re / ey_sq;
+  const double C = x0 * x0 / A;
+  const double D = y0 * y0 * ez * ez / A;
+  double sint[4];
+  const int nb = quartic_roots(-B, C + D - 1., B, -D, sint);
+  const double sx = (0. < x0) - (x0 < 0.);
+  for (int i = 0; i < 4; ++i)
+    if (i < nb && __builtin_signbit(sint[i]) != __builtin_signbit(B)) {
+      const double cost = -sx * sqrt(1. - sint[i] * sint[i]);
+      const double ptX = x0 + re * cost;
+      const double ptY = y0 + re * ez * sint[i];
+      if (ptX * ptX + ptY * ptY > rc * rc) return true;
+    }
+  return false;
+}
+
+DEM_HD d3 basis_times(const double* B, d3 v) {
+  return d3{B[0] * v.x + B[1] * v.y + B[2] * v.z, B[3] * v.x + B[4] * v.y + B[5] * v.z,
+            B[6] * v.x + B[7] * v.y + B[8] * v.z};
+}
+DEM_HD d3 unit_of(d3 v) { return v / norm(v); }
+
+// isContactBVolume(OBC, OBC) (OBC.cpp:226-363): shortest distance of the
+// axes, projections of the two rectangles, then ellipse against circle
+DEM_HD bool obc_overlap(const ConvexGeo& A, const ConvexGeo& B) {
+  double r1, hh1, r2, hh2;
+  d3 e1, e2l;
+  obc_of(A, r1, hh1, e1);
+  obc_of(B, r2, hh2, e2l);
+  const double h1 = hh1 / 2., h2 = hh2 / 2.;
+  const d3 eZ = basis_times(A.B, e1);
+  const d3 e2 = basis_times(B.B, e2l);
+  const d3 xPt = B.o - A.o;
+  const d3 eX = unit_of(cross(e2, eZ));
+  const d3 eY = unit_of(cross(eZ, eX));
+  const double x = dot(eX, xPt), y = dot(eY, xPt), z = dot(eZ, xPt);
+  const double ey = dot(eY, e2), ez = dot(eZ, e2);
+  if (fabs(x) >= r1 + r2) return false;
+  {
+    const double s2 = y / ey;
+    const double s1 = z + s2 * ez;
+    if (fabs(s1) < h1 && fabs(s2) < h2) return true;
+  }
+  {
+    const double fey = fabs(ey) + 1.e-5, fez = fabs(ez) + 1.e-5;
+    if (fabs(y) > r1 + h2 * fey + r2 * fez) return false;
+    if (fabs(z) > h1 + h2 * fez + r2 * fey) return false;
+    if (fabs(y * ez - z * ey) > r2 + h1 * fey + r1 * fez) return false;
+    if (fabs(y * ey + z * ez) > h2 + h1 * fez + r1 * fey) return false;
+  }
+  if (fabs(ez) < 0.99 && fabs(ez) > 0.01) {
+    if (obc_ellipse_apart(x, y + h2 * ey, y - h2 * ey, ez, ey, r2, r1)) return false;
+    const d3 eY2 = unit_of(cross(eX, e2));
+    const double y2 = -dot(eY2, xPt);
+    const double ey2 = dot(eY2, eZ);
+    if (obc_ellipse_apart(-x, y2 + h1 * ey2, y2 - h1 * ey2, ez, ey2, r1, r2)) return false;
+  }
+  return true;
+}
+
+DEM_HD bool bv_overlap(const ConvexGeo& A, const ConvexGeo& B) {
+  return A.opt.bv == 1 ? obb_overlap(A, B) : obc_overlap(A, B);
+}
+
+// closest_points with Johnson's sub-distance (Convex.cpp:536-627), the
+// tolerance and acceleration of A.opt; witness points in the local frames;
+// returns the distance (sep >= 0: *separated set when the distance is
+// proven larger than sep; the witness points are then not computed)
+DEM_HD double gjk_johnson_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, d3& pa,
+                                  d3& pb, int& iters, double sep, bool* separated) {
   Simplex S;
   for (int i = 0; i < 16; ++i)
     for (int j = 0; j < 4; ++j) S.det[i][j] = 0.;
@@ -337,16 +807,18 @@ DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B,
   unsigned bits = 0, all_bits = 0;
   d3 zero{0., 0., 0.};
   d3 v = tf_apply(a2w, support(A, zero)) - tf_apply(b2w, support(B, zero));
+  d3 w = v, d = v;
+  bool accel = A.opt.accel != 0;
   double dist = norm(v);
   int it = 0;
   while (bits < 15 && dist > kEps2 && it < 1000) {
     ++it;
     // the first free slot
     int go;
-    if (!(bits & 1u)) go = gjk_iter<0>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
-    else if (!(bits & 2u)) go = gjk_iter<1>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
-    else if (!(bits & 4u)) go = gjk_iter<2>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
-    else go = gjk_iter<3>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep);
+    if (!(bits & 1u)) go = gjk_iter<0>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep, it, d, w, accel);
+    else if (!(bits & 2u)) go = gjk_iter<1>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep, it, d, w, accel);
+    else if (!(bits & 4u)) go = gjk_iter<2>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep, it, d, w, accel);
+    else go = gjk_iter<3>(S, A, a2w, B, b2w, bits, all_bits, v, dist, sep, it, d, w, accel);
     if (go == 2) {
       if (separated) *separated = true;
       iters = it;
@@ -359,12 +831,23 @@ DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B,
   return dist;
 }
 
+// The GJK distance the system's options choose: the signed-volume
+// sub-distance (GJK_SV) or Johnson's; johnson: Johnson's whatever the
+// options (the ContactError recovery, RigidBodyWithCrust.cpp:545).
+DEM_HD double gjk_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w, d3& pa, d3& pb,
+                          int& iters, double sep = -1., bool* separated = nullptr, bool johnson = false) {
+  if (A.opt.sv && !johnson) return gjk_sv_closest(A, a2w, B, b2w, pa, pb, iters, sep, separated);
+  return gjk_johnson_closest(A, a2w, B, b2w, pa, pb, iters, sep, separated);
+}
+
 // General ClosestPoint path.  Returns 1 with the contact features (the
-// caller tests d < 0), 0 when the circumscribed spheres do not meet, -1 on
-// ContactError (the crust-shrunk bodies intersect).
+// caller tests d < 0), 0 when the circumscribed spheres (or, with the
+// option, the bounding volumes) do not meet, -1 on ContactError (the
+// crust-shrunk bodies intersect).
 DEM_HD int closest_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, double& d, int* iters = nullptr) {
   d3 gcagcb = A.o - B.o;
   if (!(norm(gcagcb) < A.circR + B.circR)) return 0;
+  if (A.opt.bv && !bv_overlap(A, B)) return 0;
   Tf a2w = tf_scaled(A, A.scaling), b2w = tf_scaled(B, B.scaling);
   d3 pa, pb;
   int it;
@@ -398,7 +881,7 @@ DEM_HD bool recover_gjk(const ConvexGeo& A, const ConvexGeo& B, d3& P, d3& ov, d
   Tf a2w = tf_scaled(A, sa), b2w = tf_scaled(B, sb);
   d3 pa, pb;
   int it;
-  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, it);
+  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, it, -1., nullptr, true);
   if (iters) *iters = it;
   if (distance < kEps) return false;
   pa = tf_apply(a2w, pa);
@@ -429,6 +912,7 @@ DEM_HD int gjk_attempt(const ConvexGeo& A, const ConvexGeo& B, int attempt, d3& 
   if (attempt == 0) {
     d3 gcagcb = A.o - B.o;
     if (!(norm(gcagcb) < A.circR + B.circR)) return 0;
+    if (A.opt.bv && !bv_overlap(A, B)) return 0;
   } else {
     double* as[3] = {&sa.x, &sa.y, &sa.z};
     double* bs[3] = {&sb.x, &sb.y, &sb.z};
@@ -444,7 +928,8 @@ DEM_HD int gjk_attempt(const ConvexGeo& A, const ConvexGeo& B, int attempt, d3& 
   Tf a2w = tf_scaled(A, sa), b2w = tf_scaled(B, sb);
   d3 pa, pb;
   bool separated = false;
-  double distance = gjk_closest(A, a2w, B, b2w, pa, pb, iters, early && attempt == 0 ? crusts : -1., &separated);
+  double distance =
+      gjk_closest(A, a2w, B, b2w, pa, pb, iters, early && attempt == 0 ? crusts : -1., &separated, attempt != 0);
   if (separated) return 0;
   if (distance < kEps) return -1;
   pa = tf_apply(a2w, pa);

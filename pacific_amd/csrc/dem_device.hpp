@@ -752,6 +752,7 @@ __device__ __forceinline__ ConvexGeo convex_of(const Particles& P, int s, d3 x, 
     g.bext = d3{r, r, r};
     g.crust = c;
     g.circR = r;
+    g.obb = d3{r, r, r};
   } else {
     const ShapeDesc& d = P.shapes[sh];
     g.type = d.type;
@@ -763,7 +764,9 @@ __device__ __forceinline__ ConvexGeo convex_of(const Particles& P, int s, d3 x, 
     g.bext = d3{d.bext[0], d.bext[1], d.bext[2]};
     g.crust = d.crust;
     g.circR = d.circR;
+    g.obb = d3{d.obb[0], d.obb[1], d.obb[2]};
   }
+  g.opt = P.gjk;
   return g;
 }
 
@@ -782,6 +785,8 @@ __device__ __forceinline__ ConvexGeo convex_of_obstacle(const BoxObstacle& B) {
   g.bext = g.e;
   g.crust = B.crust;
   g.circR = norm(g.e);
+  g.obb = g.e;
+  g.opt = GjkOpts{0, 0, 0, 0, 1e-10};  // the options are the particle's (body A of every pair)
   return g;
 }
 

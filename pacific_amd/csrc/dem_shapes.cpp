@@ -88,6 +88,7 @@ int build_shape(int type, double radius, const double* L, int nv, const double* 
     out.inertia[0] = out.inertia[3] = out.inertia[5] = 8.0 * kPi * std::pow(radius, 5.) / 15.0;
     d.circR = radius;
     bext[0] = bext[1] = bext[2] = radius;
+    d.obb[0] = d.obb[1] = d.obb[2] = radius;  // Sphere::computeBVolume (Sphere.cpp:528-537)
   } else if (type == kShapeBox) {
     if (!L || !(L[0] > 0.) || !(L[1] > 0.) || !(L[2] > 0.)) {
       err = "box edge lengths must be positive";
@@ -105,6 +106,7 @@ int build_shape(int type, double radius, const double* L, int nv, const double* 
     bext[0] = ex;
     bext[1] = ey;
     bext[2] = ez;
+    for (int a = 0; a < 3; ++a) d.obb[a] = d.extent[a];  // Box::computeBVolume (Box.cpp:1031-1056)
   } else if (type == kShapePoly) {
     if (nv < 4 || nf < 4 || !verts || !face_sizes || !face_idx) {
       err = "a polyhedron needs at least 4 vertices and 4 faces";
@@ -146,6 +148,11 @@ int build_shape(int type, double radius, const double* L, int nv, const double* 
     }
     d.circR = std::sqrt(ray);
     d.nv = nv;
+    // Polyhedron::computeBVolume (Polyhedron.cpp:776-820): the largest |x|,
+    // |y|, |z| over the vertices, a box centred on the reference point
+    for (int a = 0; a < 3; ++a) d.obb[a] = 0.;
+    for (int i = 0; i < nv; ++i)
+      for (int a = 0; a < 3; ++a) d.obb[a] = std::fabs(verts[3 * i + a]) > d.obb[a] ? std::fabs(verts[3 * i + a]) : d.obb[a];
     for (int a = 0; a < 3; ++a) {
       double mn = verts[a], mx = verts[a];
       for (int i = 1; i < nv; ++i) {

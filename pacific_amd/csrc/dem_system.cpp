@@ -122,6 +122,7 @@ int grains_gpu_system::allocate() {
     HIPCHECK(this, track(this, &d_verts, shape_verts.size()));
     P.shapes = d_shapes;
     P.verts = d_verts;
+    P.gjk = gjk_opts;
   }
   HIPCHECK(this, track(this, &d_obs, obstacles.size()));
   HIPCHECK(this, track(this, &d_models, (size_t)kMaxMaterials * kMaxMaterials));
@@ -826,6 +827,16 @@ int grains_gpu_add_particles(grains_gpu_system* s, int shape, size_t n, const do
   return GRAINS_OK;
 }
 
+int grains_gpu_set_collision(grains_gpu_system* s, int gjk_sv, double tolerance, int acceleration,
+                             int bounding_volume) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  if (s->initialized) return fail(s, GRAINS_ARGUMENT_ERROR, "collision options are set before initialisation");
+  if (!(tolerance >= 1e-10)) return fail(s, GRAINS_CONFIG_ERROR, "GJK tolerance should be at least 1E-10");
+  if (bounding_volume < 0 || bounding_volume > 2) return fail(s, GRAINS_ARGUMENT_ERROR, "bounding volume 0, 1 or 2");
+  s->gjk_opts = dem::GjkOpts{gjk_sv ? 1 : 0, acceleration ? 1 : 0, bounding_volume, 0, tolerance};
+  return GRAINS_OK;
+}
+
 int grains_gpu_closest_point(grains_gpu_system* s, int shapeA, const double* xA, const double* BA, int shapeB,
                              const double* xB, const double* BB, double* out) {
   if (!s || !xA || !BA || !xB || !BB || !out) return GRAINS_ARGUMENT_ERROR;
@@ -845,6 +856,8 @@ int grains_gpu_closest_point(grains_gpu_system* s, int shapeA, const double* xA,
     g.bext = d3{d.bext[0], d.bext[1], d.bext[2]};
     g.crust = d.crust;
     g.circR = d.circR;
+    g.obb = d3{d.obb[0], d.obb[1], d.obb[2]};
+    g.opt = s->gjk_opts;
     return g;
   };
   ConvexGeo A = geo(shapeA, xA, BA), B = geo(shapeB, xB, BB);

@@ -83,6 +83,7 @@ struct grains_gpu_system {
   // per-step neighbour candidate list (0 = the force sweep searches the 27
   // cells itself); set before initialisation
   int nbr_cap = 32;
+  dem::GjkOpts gjk_opts{0, 0, 0, 0, 1e-10};  // <CollisionDetectionAlgorithm>
   int* d_nbr_count = nullptr;
   unsigned* d_nbr = nullptr;
   dem::NbrList nbr_list() const { return dem::NbrList{d_nbr_count, d_nbr, nbr_cap, nparticles}; }

@@ -72,6 +72,16 @@ struct ShapeDesc {
   double scaling[3];  // (bbox extent - crust) / bbox extent per axis
   double bext[3];     // bounding-box half extents at the identity transform
   double crust, circR;
+  double obb[3];      // OBB half extents (Box/Polyhedron/Sphere::computeBVolume)
+};
+
+// <CollisionDetectionAlgorithm> (Grains.cpp:640-770): GJK with Johnson's
+// (sv 0) or the signed-volume (sv 1, GJK_SV) sub-distance, relative
+// tolerance of the optimality gap, Nesterov acceleration, and the
+// bounding-volume pre-test of general pairs (bv 0 none, 1 OBB, 2 OBC).
+struct GjkOpts {
+  int sv, accel, bv, pad;
+  double tol;
 };
 
 // Particle state.  x, v, w are double buffered (index cur / 1-cur) because the
@@ -107,6 +117,7 @@ struct Particles {
   double* I3[3];
   const ShapeDesc* shapes;
   const double* verts;
+  GjkOpts gjk;  // narrow phase options of the system's general pairs
 };
 
 // Cell-sorted view rebuilt every step by the binning kernels.

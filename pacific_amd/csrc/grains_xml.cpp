@@ -409,6 +409,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
     c.bad("only <Grains3D Type=\"Standard\"> is supported");
   cfg.fluid_density = opt.fluid_density;
   const XNode* cons = c.ok ? root.child("Construction") : nullptr;
+  dem::GjkOpts gjk{0, 0, 0, 0, 1e-10};
   if (c.ok && !cons) c.bad("missing <Construction>");
 
   // ---- Construction (Grains.cpp:535-1033)
@@ -428,30 +429,38 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
       for (int a = 0; a < 3; ++a) cfg.periodic[a] = std::atoi(p->attr(ax[a]).c_str()) == 1 ? 1 : 0;
     }
     if (cons->child("PartialPeriodicity")) c.bad("<PartialPeriodicity> is not supported");
-    // <CollisionDetectionAlgorithm> (Grains.cpp:640-770): Johnson's GJK at
-    // the default relative tolerance 1e-10, without acceleration or history,
-    // is what this library runs; other choices are refused rather than
-    // ignored.  A <BoundingVolume> (OBB / OBC) is only a pre-test that skips
-    // GJK for separated pairs, so the contacts are the same without it.
-    if (const XNode* cda = cons->child("CollisionDetectionAlgorithm"))
+    // <CollisionDetectionAlgorithm> (Grains.cpp:640-770): <CollisionDetection
+    // Type="GJK|GJK_SV" Tolerance Acceleration="ON|OFF" History="OFF"> and
+    // <BoundingVolume Type="OBB|OBC">.  History="ON" (the warm-started GJK of
+    // Convex.cpp:636-733) is refused rather than ignored.
+    if (const XNode* cda = cons->child("CollisionDetectionAlgorithm")) {
       if (const XNode* cd = cda->child("CollisionDetection")) {
         if (cd->has("Type")) {
           const std::string t = cd->attr("Type");
-          if (t == "GJK_SV") c.bad("<CollisionDetection Type=\"GJK_SV\"> is not supported (GJK only)");
+          if (t == "GJK_SV") gjk.sv = 1;
           else if (t != "GJK") c.bad("collision detection algorithm " + t + " is not defined");
         }
         if (c.ok && cd->has("Tolerance")) {
           const double tol = dattr(cd, "Tolerance", c);
           if (c.ok && tol < 1e-10) c.bad("GJK tolerance should be at least 1E-10");
-          else if (c.ok && tol != 1e-10) c.bad("only the default GJK tolerance 1e-10 is supported");
+          gjk.tol = tol;
         }
-        for (const char* opt_name : {"Acceleration", "History"}) {
-          if (!c.ok || !cd->has(opt_name)) continue;
-          const std::string v = cd->attr(opt_name);
-          if (v == "ON") c.bad(std::string("GJK ") + opt_name + "=\"ON\" is not supported");
-          else if (v != "OFF") c.bad(std::string(opt_name) + " should be ON or OFF");
+        if (c.ok && cd->has("Acceleration")) {
+          const std::string v = cd->attr("Acceleration");
+          if (v == "ON") gjk.accel = 1;
+          else if (v != "OFF") c.bad("Acceleration should be ON or OFF");
+        }
+        if (c.ok && cd->has("History")) {
+          const std::string v = cd->attr("History");
+          if (v == "ON") c.bad("GJK History=\"ON\" is not supported");
+          else if (v != "OFF") c.bad("History should be ON or OFF");
         }
       }
+      if (const XNode* bvn = cda->child("BoundingVolume")) {
+        const std::string t = bvn->attr("Type");
+        gjk.bv = t == "OBB" ? 1 : (t == "OBC" ? 2 : 0);
+      }
+    }
     // <Reload Type="new" Filename="..."> or <Reload Type="same"> (the last
     // file of <RestartFile Name>_RFTable.txt), Velocity="Reset" optional
     // (Grains.cpp:781-844): the particles the input file inserts take the
@@ -728,6 +737,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
     s->last_error = c.err;
     return GRAINS_CONFIG_ERROR;
   }
+  s->gjk_opts = gjk;
   s->t_end = t_end;
   s->save_times = save_times;
   s->save_root = save_root;

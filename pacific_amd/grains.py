@@ -52,6 +52,7 @@ EXPORTS = [
     "grains_gpu_graph_stats", "grains_gpu_force_sums", "grains_gpu_set_hydro_ft",
     "grains_gpu_coordination_at_state", "grains_gpu_post_processing", "grains_gpu_save_reload",
     "grains_gpu_load_reload", "grains_gpu_reload_same", "grains_gpu_force_sums_noalloc",
+    "grains_gpu_set_collision",
 ]
 
 
@@ -80,6 +81,7 @@ def lib():
         L.grains_gpu_compute_forces.argtypes = [vp]
         L.grains_gpu_force_sums.argtypes = [vp, dp]
         L.grains_gpu_force_sums_noalloc.argtypes = [vp, vp]
+        L.grains_gpu_set_collision.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.grains_gpu_get_state.argtypes = [vp] + [dp] * 7
         L.grains_gpu_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
         L.grains_gpu_get_masses.argtypes = [vp, dp, dp, dp]
@@ -297,6 +299,13 @@ class GrainsGPU:
 
     def contact_model(self, matA, matB, kn, en, muc, kr=0.0, etat=-1.0):
         self._check(lib().grains_gpu_set_contact_model(self.h, matA, matB, kn, en, etat, muc, kr))
+
+    def collision(self, gjk_sv=False, tolerance=1e-10, acceleration=False, bounding_volume=0):
+        """<CollisionDetectionAlgorithm>: GJK_SV or Johnson, relative
+        tolerance, Nesterov acceleration, pre-test 0 none / 1 OBB / 2 OBC
+        (before init)."""
+        self._check(lib().grains_gpu_set_collision(self.h, int(bool(gjk_sv)), float(tolerance),
+                                                   int(bool(acceleration)), int(bounding_volume)))
 
     def memory_contact_model(self, matA, matB, kn, en, muc, kt, etat=-1.0, mur=0.0, etarpf=0.0, epst=0.0):
         """HookeMemory contact model (tangential spring kt, rolling friction mur)."""

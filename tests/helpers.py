@@ -6,22 +6,30 @@ from oracle.oracle import Oracle
 from pacific_amd import workloads
 
 
-def make_oracle(wl, scheme=0, fluid_density=0.0):
+def make_oracle(wl, scheme=0, fluid_density=0.0, collision=None):
+    """collision: keyword arguments of Oracle.collision (GJK_SV, tolerance,
+    acceleration, bounding volume), set before the initial forces."""
     o = Oracle(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], fluid_density,
                wl["dt"], scheme)
     workloads.build(o, wl)
+    if collision:
+        o.collision(**collision)
     o.init()
     return o
 
 
-def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None, skin=None, stats=False):
+def make_gpu(wl, scheme=0, precision=0, fluid_density=0.0, record=True, nbr=None, skin=None, stats=False,
+             collision=None):
     """nbr: candidates per particle (0: cell search in the force sweep);
     skin: Verlet skin (0: per-step list); None: the library defaults;
-    stats: collect contact statistics from the first force evaluation."""
+    stats: collect contact statistics from the first force evaluation;
+    collision: keyword arguments of GrainsGPU.collision."""
     from pacific_amd import GrainsGPU
     g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], fluid_density,
                   wl["dt"], scheme, precision)
     workloads.build(g, wl)
+    if collision:
+        g.collision(**collision)
     if nbr is not None or skin is not None:
         g.neighbor_list(32 if nbr is None else nbr, -1.0 if skin is None else skin)
     if record:

@@ -80,3 +80,44 @@ def test_exchange_symmetry(both, ka, kb):
                 assert np.abs(P1 - P2).max() <= 4 * np.spacing(np.abs(P1).max()), k
             else:
                 assert np.array_equal(P1, P2), (k, P1 - P2)
+
+
+OPTIONS = [dict(gjk_sv=True), dict(tolerance=1e-6), dict(acceleration=True), dict(gjk_sv=True, acceleration=True),
+           dict(bounding_volume=1), dict(bounding_volume=2), dict(gjk_sv=True, bounding_volume=1)]
+
+
+@pytest.mark.parametrize("opts", OPTIONS, ids=lambda d: "-".join(f"{k}={v}" for k, v in d.items()))
+def test_collision_options_equal_oracle(opts):
+    """<CollisionDetectionAlgorithm> options (GJK_SV, Tolerance,
+    Acceleration, BoundingVolume OBB / OBC) on both sides: the same bits."""
+    o = Oracle((0, 0, 0), (1, 1, 1), (0, 0, 0), 1.0, (0, 0, 0), 0.0, 1e-5, 0)
+    g = GrainsGPU((0, 0, 0), (1, 1, 1))
+    ids = {}
+    for name in KINDS:
+        if name == "box":
+            a, b = o.add_shape(1, L=(2e-3, 1.6e-3, 1.2e-3)), g.add_shape(1, L=(2e-3, 1.6e-3, 1.2e-3))
+        elif name == "sph":
+            a, b = o.add_shape(0, radius=1e-3), g.add_shape(0, radius=1e-3)
+        else:
+            v, f = shapes.tetrahedron(1.2e-3) if name == "tet" else shapes.icosahedron(1.2e-3)
+            a, b = o.add_shape(2, verts=v, faces=f), g.add_shape(2, verts=v, faces=f)
+        ids[name] = a
+    o.collision(**opts)
+    g.collision(**opts)
+    rng = np.random.default_rng(11)
+    status = {}
+    for ka in KINDS:
+        for kb in KINDS:
+            for k in range(60):
+                BA, BB = shapes.random_rotations(2, 5000 + 100 * KINDS.index(ka) + 10 * KINDS.index(kb) + k)
+                u = rng.normal(size=3)
+                u /= np.linalg.norm(u)
+                xA = rng.uniform(-1e-3, 1e-3, 3)
+                xB = xA + u * rng.uniform(1.0e-3, 3.2e-3)
+                Po, ovo, do, so = o.closest_point(ids[ka], xA, BA, ids[kb], xB, BB)
+                Pg, ovg, dg, sg = g.closest_point(ids[ka], xA, BA, ids[kb], xB, BB)
+                assert so == sg, (ka, kb, k, so, sg)
+                status[so] = status.get(so, 0) + 1
+                if so in (1, 2):
+                    assert np.array_equal(Po, Pg) and np.array_equal(ovo, ovg) and do == dg, (ka, kb, k)
+    assert status.get(1, 0) > 150 and status.get(0, 0) > 100

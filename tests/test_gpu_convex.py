@@ -79,3 +79,30 @@ def test_polyhedra_short_rollout(gpu):
     so, sg = o.state(), g.state()
     assert np.abs(sg["x"] - so["x"]).max() < 1e-9
     assert np.abs(sg["v"] - so["v"]).max() < 1e-6 * max(np.abs(so["v"]).max(), 1e-3)
+
+
+@pytest.mark.parametrize("opts", [dict(gjk_sv=True), dict(bounding_volume=1),
+                                  dict(bounding_volume=2, acceleration=True), dict(gjk_sv=True, tolerance=1e-8)],
+                         ids=["gjk_sv", "obb", "obc_accel", "gjk_sv_tol"])
+def test_collision_options_pairs_and_forces(gpu, opts):
+    """<CollisionDetectionAlgorithm> options on a settling packing of
+    cubes, tetrahedra and icosahedra: the oracle's pair set and forces
+    within 1e-12 after 12,000 oracle steps (the narrow phase itself is
+    bit-identical, tests/test_convex_narrow_phase.py), and a short rollout
+    within the bars of test_polyhedra_short_rollout."""
+    wl = workloads.polyhedra_box((4, 4, 3), seed=13, sigma=0.2, en=0.3)
+    o = make_oracle(wl, collision=opts)
+    o.step(12000)
+    g = make_gpu(wl, collision=opts)
+    transfer(o, g)
+    po, pg = pair_set(o.pairs()), pair_set(g.pairs())
+    assert len(po) > 40
+    assert pg == po
+    so, sg = o.state(), g.state()
+    _, _, wgt = o.masses()
+    scale = contact_scale(so, wgt)
+    assert rel_force_error(sg["F"], so["F"], scale) <= 1e-12
+    o.step(300)
+    g.step(300)
+    so, sg = o.state(), g.state()
+    assert np.abs(sg["x"] - so["x"]).max() < 1e-9

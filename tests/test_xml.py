@@ -120,15 +120,31 @@ def test_valid_file_parses(tmp_path):
     (("<TimeStep dt=\"1.e-5\"/>", "<TimeStep/>"), "missing attribute dt"),
     (("</Construction>", "</Construktion>"), "does not match"),
     (("positions.result", "missing.result"), "cannot open position file"),
-    (('Type="GJK"', 'Type="GJK_SV"'), "GJK_SV"),
-    (('Tolerance="1.e-10"', 'Tolerance="1.e-6"'), "default GJK tolerance"),
+    (('Type="GJK"', 'Type="EPA"'), "not defined"),
     (('Tolerance="1.e-10"', 'Tolerance="1.e-12"'), "at least 1E-10"),
-    (('Tolerance="1.e-10"', 'Tolerance="1.e-10" Acceleration="ON"'), "Acceleration"),
+    (('Tolerance="1.e-10"', 'Tolerance="1.e-10" Acceleration="maybe"'), "ON or OFF"),
+    (('Tolerance="1.e-10"', 'Tolerance="1.e-10" History="ON"'), "History"),
     (('Tolerance="1.e-10"', 'Tolerance="1.e-10" History="maybe"'), "ON or OFF"),
 ])
 def test_rejections(tmp_path, sub, frag):
     path, _ = write_case(tmp_path, **dict(zip(sub[::2], sub[1::2])))
     expect_error(path, "ConfigError", frag)
+
+
+@pytest.mark.parametrize("opts", [
+    '<CollisionDetection Type="GJK_SV" Tolerance="1.e-8" Acceleration="ON" History="OFF"/>',
+    '<CollisionDetection Type="GJK" Tolerance="1.e-10"/>\n      <BoundingVolume Type="OBB"/>',
+    '<CollisionDetection Type="GJK_SV"/>\n      <BoundingVolume Type="OBC"/>',
+])
+def test_collision_options_accepted(tmp_path, opts):
+    # <CollisionDetectionAlgorithm> (Grains.cpp:640-770): GJK_SV, Tolerance,
+    # Acceleration and the OBB / OBC pre-tests are accepted
+    path, _ = write_case(tmp_path, **{'<CollisionDetection Type="GJK" Tolerance="1.e-10"/>': opts})
+    assert opts in open(path).read()
+    if gpu_available():
+        GrainsGPU.from_xml(path)
+    else:
+        expect_error(path, "DeviceError", "")
 
 
 def test_missing_contact_model(tmp_path):
