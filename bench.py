@@ -43,12 +43,14 @@ NBR_CAP = 32  # neighbour candidates per particle (0: the force sweep searches t
 SKIN = -1.0  # Verlet skin (< 0: the library default, a tenth of the cell size; 0: per-step list)
 
 
-def build_gpu(wl, precision, skin=None):
+def build_gpu(wl, precision, skin=None, collision=None):
     from pacific_amd import GrainsGPU, workloads
     g = GrainsGPU(wl["origin"], wl["max"], wl["periodic"], wl["cellsize_factor"], wl["gravity"], 0.0, wl["dt"],
                   0, precision)
     workloads.build(g, wl)
     g.neighbor_list(NBR_CAP, SKIN if skin is None else skin)
+    if collision:
+        g.collision(**collision)
     g.init()
     return g
 
@@ -359,7 +361,8 @@ def polyhedra_leg(args, prec):
     composition."""
     from pacific_amd import FP64, workloads
     wl = workloads.silo(args.poly_n)
-    g = build_gpu(wl, prec)
+    col = {"gjk_sv": True} if args.poly_gjk_sv else None
+    g = build_gpu(wl, prec, collision=col)
     t0 = time.perf_counter()
     done = 0
     if args.poly_state and os.path.exists(args.poly_state):
@@ -396,7 +399,7 @@ def polyhedra_leg(args, prec):
     detect_ms = prof["detect_ms"] / max(prof["n_detect"], 1)
     ferr = None
     if prec != FP64:
-        g64 = build_gpu(wl, FP64)
+        g64 = build_gpu(wl, FP64, collision=col)
         ferr = force_error(g, g64, shapes=True)
         del g64
         log(f"bench: polyhedra mixed vs FP64 force error {ferr.get('max_rel_per_particle_sum', 0.0):.2e} "
@@ -409,6 +412,7 @@ def polyhedra_leg(args, prec):
             "polyhedra_config": {"workload": f"silo of {args.poly_n} polyhedra (cube 2 mm, tetrahedron and "
                                              "icosahedron R 1.2 mm), random orientations, released from rest, "
                                              "GJK, dt 5e-6", "fill_steps": args.poly_fill,
+                                 "gjk": "GJK_SV" if args.poly_gjk_sv else "Johnson",
                                  "warmup": args.poly_warmup, "steps": args.poly_steps,
                                  "ms_per_step": ms / args.poly_steps, "narrow_phase_ms": narrow_ms,
                                  "force_sweep_ms": force_ms, "candidate_search_ms": detect_ms,
@@ -514,6 +518,8 @@ def main():
     ap.add_argument("--poly-fill", type=int, default=20000, help="untimed steps of the silo fill from rest")
     ap.add_argument("--poly-state", default="", metavar="NPZ",
                     help="load the filled silo from this file if it exists, else save it there after the fill")
+    ap.add_argument("--poly-gjk-sv", action="store_true",
+                    help="the polyhedra leg with <CollisionDetection Type=\"GJK_SV\"> (default: Johnson's GJK)")
     ap.add_argument("--only-poly", action="store_true", help="run the polyhedra leg alone (profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--only-adjoint", action="store_true", help="run the adjoint leg alone (profiling)")

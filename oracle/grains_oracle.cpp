@@ -684,7 +684,7 @@ bool sphereBoxA(Geo const& A, Geo const& B, Contact& c) {
   return true;
 }
 
-long g_gjk_calls = 0, g_gjk_iters = 0;
+long g_gjk_calls = 0, g_gjk_iters = 0, g_gjk_last = 0;
 
 // General path (RigidBodyWithCrust.cpp:277-347); false on ContactError
 bool closestPointGJK(Geo const& A, Geo const& B, Contact& c) {
@@ -708,6 +708,7 @@ bool closestPointGJK(Geo const& A, Geo const& B, Contact& c) {
                              : gjkClosestPoints(A.c, B.c, a2w, b2w, pa, pb, nb);
   ++g_gjk_calls;
   g_gjk_iters += nb;
+  g_gjk_last = nb;
   c.nit = nb;
   if (distance < EPSILON) return false;
   pa = applyTf(a2w, pa);
@@ -1763,6 +1764,8 @@ void orc_gjk_stats(long* calls, long* iters) {
   *calls = g_gjk_calls;
   *iters = g_gjk_iters;
 }
+// iterations of the last GJK call
+long orc_gjk_last() { return g_gjk_last; }
 
 // n spheres at once (same construction as orc_add_sphere, no basis)
 int orc_add_spheres(void* h, int n, const double* radius, const double* density, const double* crust,

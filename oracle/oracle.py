@@ -69,6 +69,7 @@ def lib():
         L.orc_set_basis.argtypes = [vp, dp]
         L.orc_closest_point.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, dp]
         L.orc_gjk_stats.argtypes = [ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
+        L.orc_gjk_last.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -214,6 +215,11 @@ class Oracle:
                                 _d(np.asarray(BA, np.float64).reshape(9)), int(shapeB),
                                 _d(np.asarray(xB, np.float64)), _d(np.asarray(BB, np.float64).reshape(9)), _ptr(out))
         return out[0:3].copy(), out[3:6].copy(), out[6], int(out[7])
+
+    @staticmethod
+    def gjk_last_iterations():
+        """Iterations of the last GJK distance computation (either algorithm)."""
+        return lib().orc_gjk_last()
 
     def set_basis(self, B):
         lib().orc_set_basis(self.h, _d(np.asarray(B, np.float64).reshape(self.n, 9)))

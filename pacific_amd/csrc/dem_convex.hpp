@@ -377,12 +377,6 @@ struct SvSimplex {
 };
 
 DEM_HD bool sv_same_sign(double a, double b) { return !((a > 0) ^ (b > 0)); }
-// component C of v, C a compile-time constant
-template <int C>
-DEM_HD double cget(const d3& v) {
-  return C == 0 ? v.x : (C == 1 ? v.y : v.z);
-}
-
 // sum of lam_i y_i over the slots of bits, in slot order
 DEM_HD d3 sv_point(const d3 (&y)[4], unsigned bits, const L4& l) {
   d3 v{0., 0., 0.};
@@ -393,25 +387,28 @@ DEM_HD d3 sv_point(const d3 (&y)[4], unsigned bits, const L4& l) {
   return v;
 }
 
-template <int I1, int I2, int I>
-DEM_HD void sv_s1d_tail(const d3 (&y)[4], const d3& t, double neg_tI, unsigned& bits, L4& lam) {
-  const double pI = dot(y[I2], t) / dot(t, t) * neg_tI + cget<I>(y[I2]);
-  const double C0 = -cget<I>(y[I2]) + pI, C1 = cget<I>(y[I1]) - pI;
-  const bool s0 = sv_same_sign(neg_tI, C0), s1 = sv_same_sign(neg_tI, C1);
-  // both weights written in every case (values selected, not addresses:
-  // a store through a selected address would put the weights in scratch)
-  const bool both = s0 && s1;
-  const double l1 = both ? C0 / neg_tI : (s0 ? 1. : lw<I1>(lam));
-  const double l2 = both ? C1 / neg_tI : (s0 ? lw<I2>(lam) : 1.);
-  lw<I1>(lam) = l1;
-  lw<I2>(lam) = l2;
-  if (!both) bits &= ~(1u << (s0 ? I2 : I1));
+// component i of v (0, 1, 2) selected among the values
+DEM_HD double sel3(int i, d3 v) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+// c ? a : b component by component (a select of values: a select of the
+// addresses of a and b would keep them in scratch memory)
+DEM_HD d3 pick(bool c, d3 a, d3 b) { return d3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+// a and b equal bit for bit (signed zeros told apart)
+DEM_HD bool same_bits(d3 a, d3 b) {
+  return __builtin_bit_cast(unsigned long long, a.x) == __builtin_bit_cast(unsigned long long, b.x) &&
+         __builtin_bit_cast(unsigned long long, a.y) == __builtin_bit_cast(unsigned long long, b.y) &&
+         __builtin_bit_cast(unsigned long long, a.z) == __builtin_bit_cast(unsigned long long, b.z);
 }
 
-// the point of the edge I1-I2 closest to the origin (GJK_SV.cpp:19-93)
-template <int I1, int I2>
-DEM_HD void sv_s1d(const d3 (&y)[4], unsigned& bits, L4& lam) {
-  const d3 t = y[I2] - y[I1];
+// The sub-distance works on the active points gathered in slot order into
+// named values (a, b, c, ...): one copy of the edge, triangle and
+// tetrahedron code each, the faces (edges) to recurse into picked by a loop
+// that is not unrolled.  The arithmetic is the slot-indexed one's, in the
+// same order.  Each returns the mask of the points it keeps (bit k = k-th
+// point) and writes their weights; a dropped point's weight is undefined.
+
+// the point of the edge a-b closest to the origin (GJK_SV.cpp:19-93)
+DEM_HD unsigned sv_edge(const d3& a, const d3& b, double& la, double& lb) {
+  const d3 t = b - a;
   int I = 0;
   double neg_tI = -t.x;
   if (fabs(t.y) > fabs(neg_tI)) {
@@ -422,86 +419,77 @@ DEM_HD void sv_s1d(const d3 (&y)[4], unsigned& bits, L4& lam) {
     I = 2;
     neg_tI = -t.z;
   }
-  if (I == 0) sv_s1d_tail<I1, I2, 0>(y, t, neg_tI, bits, lam);
-  else if (I == 1) sv_s1d_tail<I1, I2, 1>(y, t, neg_tI, bits, lam);
-  else sv_s1d_tail<I1, I2, 2>(y, t, neg_tI, bits, lam);
+  const double y2I = sel3(I, b), y1I = sel3(I, a);
+  const double pI = dot(b, t) / dot(t, t) * neg_tI + y2I;
+  const double C0 = -y2I + pI, C1 = y1I - pI;
+  const bool s0 = sv_same_sign(neg_tI, C0), s1 = sv_same_sign(neg_tI, C1);
+  const bool both = s0 && s1;
+  la = both ? C0 / neg_tI : 1.;
+  lb = both ? C1 / neg_tI : 1.;
+  return both ? 3u : (s0 ? 1u : 2u);
 }
 
-template <int I0, int I1, int I2>
-DEM_HD void sv_s2d(const d3 (&y)[4], unsigned& bits, L4& lam);
-
-// The recursion into the sub-simplex without slot J (edges of a triangle,
-// faces of a tetrahedron) keeps it when its point is the closest so far
-// (GJK_SV.cpp:190-225, 338-370)
-template <int J, int A0, int A1, int A2 = -1>
-DEM_HD void sv_keep(const d3 (&y)[4], unsigned bits, double& dbest, unsigned& best_bits, L4& lam) {
-  unsigned used = bits & ~(1u << J);
-  L4 nl{0., 0., 0., 0.};
-  if constexpr (A2 < 0) sv_s1d<A0, A1>(y, used, nl);
-  else sv_s2d<A0, A1, A2>(y, used, nl);
-  const d3 np = sv_point(y, used, nl);
-  const double ds = dot(np, np);
-  if (ds < dbest) {
-    best_bits = used;
-    dbest = ds;
-    lam = nl;
-  }
-}
-
-// signed areas of the sub-triangles with the projected origin p0 in the
-// plane of axes X, Y (GJK_SV.cpp:161-188)
-template <int X, int Y>
-DEM_HD void sv_areas(d3 p0, d3 a, d3 b, d3 c, double& C0, double& C1, double& C2) {
-  const double px = cget<X>(p0), py = cget<Y>(p0);
-  const double y0x = cget<X>(a), y0y = cget<Y>(a), y1x = cget<X>(b), y1y = cget<Y>(b), y2x = cget<X>(c),
-               y2y = cget<Y>(c);
-  C0 = (px * y1y + py * y2x + y1x * y2y - px * y2y - py * y1x - y2x * y1y);
-  C1 = (px * y2y + py * y0x + y2x * y0y - px * y0y - py * y2x - y0x * y2y);
-  C2 = (px * y0y + py * y1x + y0x * y1y - px * y1y - py * y0x - y1x * y0y);
-}
-
-// the triangle I0-I1-I2 (GJK_SV.cpp:99-228)
-template <int I0, int I1, int I2>
-DEM_HD void sv_s2d(const d3 (&y)[4], unsigned& bits, L4& lam) {
-  const d3 n = cross(y[I1] - y[I0], y[I2] - y[I0]);
-  const d3 p0 = (dot(y[I0], n) / dot(n, n)) * n;
+// the triangle a-b-c (GJK_SV.cpp:99-228); the edges without a point whose
+// signed area disagrees in sign are tried in point order, the closest kept
+// (GJK_SV.cpp:190-225)
+DEM_HD unsigned sv_tri(const d3& a, const d3& b, const d3& c, double& la, double& lb, double& lc) {
+  const d3 n = cross(b - a, c - a);
+  const d3 p0 = (dot(a, n) / dot(n, n)) * n;
   int ix = 1, iy = 2;
-  double mu_max = (y[I1].y * y[I2].z + y[I0].y * y[I1].z + y[I2].y * y[I0].z - y[I1].y * y[I0].z -
-                   y[I2].y * y[I1].z - y[I0].y * y[I2].z);
-  double mu = (y[I1].z * y[I0].x + y[I2].z * y[I1].x + y[I0].z * y[I2].x - y[I1].z * y[I2].x -
-               y[I0].z * y[I1].x - y[I2].z * y[I0].x);
+  double mu_max = (b.y * c.z + a.y * b.z + c.y * a.z - b.y * a.z - c.y * b.z - a.y * c.z);
+  double mu = (b.z * a.x + c.z * b.x + a.z * c.x - b.z * c.x - a.z * b.x - c.z * a.x);
   if (fabs(mu) > fabs(mu_max)) {
     mu_max = mu;
     ix = 0;
   }
-  mu = (y[I1].x * y[I2].y + y[I0].x * y[I1].y + y[I2].x * y[I0].y - y[I1].x * y[I0].y - y[I2].x * y[I1].y -
-        y[I0].x * y[I2].y);
+  mu = (b.x * c.y + a.x * b.y + c.x * a.y - b.x * a.y - c.x * b.y - a.x * c.y);
   if (fabs(mu) > fabs(mu_max)) {
     mu_max = mu;
     ix = 0;
     iy = 1;
   }
-  double C0, C1, C2;
-  if (ix == 1) sv_areas<1, 2>(p0, y[I0], y[I1], y[I2], C0, C1, C2);
-  else if (iy == 2) sv_areas<0, 2>(p0, y[I0], y[I1], y[I2], C0, C1, C2);
-  else sv_areas<0, 1>(p0, y[I0], y[I1], y[I2], C0, C1, C2);
+  const double px = sel3(ix, p0), py = sel3(iy, p0);
+  const double y0x = sel3(ix, a), y0y = sel3(iy, a), y1x = sel3(ix, b), y1y = sel3(iy, b), y2x = sel3(ix, c),
+               y2y = sel3(iy, c);
+  const double C0 = (px * y1y + py * y2x + y1x * y2y - px * y2y - py * y1x - y2x * y1y);
+  const double C1 = (px * y2y + py * y0x + y2x * y0y - px * y0y - py * y2x - y0x * y2y);
+  const double C2 = (px * y0y + py * y1x + y0x * y1y - px * y1y - py * y0x - y1x * y0y);
   const bool s0 = sv_same_sign(mu_max, C0), s1 = sv_same_sign(mu_max, C1), s2 = sv_same_sign(mu_max, C2);
   if (s0 && s1 && s2) {
-    lw<I0>(lam) = C0 / mu_max;
-    lw<I1>(lam) = C1 / mu_max;
-    lw<I2>(lam) = C2 / mu_max;
-    return;
+    la = C0 / mu_max;
+    lb = C1 / mu_max;
+    lc = C2 / mu_max;
+    return 7u;
   }
+  la = lb = lc = 0.;
   double dbest = 1.e9;
-  unsigned best_bits = 0;
-  if (!s0) sv_keep<I0, I1, I2>(y, bits, dbest, best_bits, lam);
-  if (!s1) sv_keep<I1, I0, I2>(y, bits, dbest, best_bits, lam);
-  if (!s2) sv_keep<I2, I0, I1>(y, bits, dbest, best_bits, lam);
-  bits = best_bits;
+  unsigned best = 0;
+#pragma nounroll
+  for (int j = 0; j < 3; ++j) {
+    if (j == 0 ? s0 : (j == 1 ? s1 : s2)) continue;
+    // the edge without point j, its points in order
+    const d3 e0 = pick(j == 0, b, a), e1 = pick(j == 2, b, c);
+    double w0, w1;
+    const unsigned k = sv_edge(e0, e1, w0, w1);
+    d3 np{0., 0., 0.};
+    if (k & 1u) np += w0 * e0;
+    if (k & 2u) np += w1 * e1;
+    const double ds = dot(np, np);
+    if (ds < dbest) {
+      dbest = ds;
+      const unsigned m0 = j == 0 ? 2u : 1u, m1 = j == 2 ? 2u : 4u;
+      best = ((k & 1u) ? m0 : 0u) | ((k & 2u) ? m1 : 0u);
+      la = j == 0 ? la : w0;
+      lb = j == 0 ? w0 : (j == 2 ? w1 : lb);
+      lc = j == 2 ? lc : w1;
+    }
+  }
+  return best;
 }
 
-// the tetrahedron (GJK_SV.cpp:234-373)
-DEM_HD void sv_s3d(const d3 (&y)[4], unsigned& bits, L4& lam) {
+// the tetrahedron y0-y1-y2-y3 (GJK_SV.cpp:234-373), its faces tried in
+// point order as the triangle's edges (GJK_SV.cpp:338-370)
+DEM_HD unsigned sv_tet(const d3 (&y)[4], L4& lam) {
   const double C0 = y[3].x * y[2].y * y[1].z + y[2].x * y[1].y * y[3].z + y[1].x * y[3].y * y[2].z -
                     y[1].x * y[2].y * y[3].z - y[2].x * y[3].y * y[1].z - y[3].x * y[1].y * y[2].z;
   const double C1 = y[0].x * y[2].y * y[3].z + y[2].x * y[3].y * y[0].z + y[3].x * y[0].y * y[2].z -
@@ -518,45 +506,79 @@ DEM_HD void sv_s3d(const d3 (&y)[4], unsigned& bits, L4& lam) {
     lam.w1 = C1 / dM;
     lam.w2 = C2 / dM;
     lam.w3 = C3 / dM;
-    return;
+    return 15u;
   }
   double dbest = 1.e9;
-  unsigned best_bits = 0;
-  if (!s0) sv_keep<0, 1, 2, 3>(y, bits, dbest, best_bits, lam);
-  if (!s1) sv_keep<1, 0, 2, 3>(y, bits, dbest, best_bits, lam);
-  if (!s2) sv_keep<2, 0, 1, 3>(y, bits, dbest, best_bits, lam);
-  if (!s3) sv_keep<3, 0, 1, 2>(y, bits, dbest, best_bits, lam);
-  bits = best_bits;
+  unsigned best = 0;
+#pragma nounroll
+  for (int j = 0; j < 4; ++j) {
+    if (j == 0 ? s0 : (j == 1 ? s1 : (j == 2 ? s2 : s3))) continue;
+    // the face without point j: points q0 < q1 < q2
+    const int q0 = j == 0 ? 1 : 0, q1 = j <= 1 ? 2 : 1, q2 = j <= 2 ? 3 : 2;
+    const d3 f0 = pick(j == 0, y[1], y[0]), f1 = pick(j <= 1, y[2], y[1]), f2 = pick(j <= 2, y[3], y[2]);
+    double w0, w1, w2;
+    const unsigned k = sv_tri(f0, f1, f2, w0, w1, w2);
+    d3 np{0., 0., 0.};
+    if (k & 1u) np += w0 * f0;
+    if (k & 2u) np += w1 * f1;
+    if (k & 4u) np += w2 * f2;
+    const double ds = dot(np, np);
+    if (ds < dbest) {
+      dbest = ds;
+      best = ((k & 1u) ? 1u << q0 : 0u) | ((k & 2u) ? 1u << q1 : 0u) | ((k & 4u) ? 1u << q2 : 0u);
+      lam.w0 = q0 == 0 ? w0 : lam.w0;
+      lam.w1 = q0 == 1 ? w0 : (q1 == 1 ? w1 : lam.w1);
+      lam.w2 = q1 == 2 ? w1 : (q2 == 2 ? w2 : lam.w2);
+      lam.w3 = q2 == 3 ? w2 : lam.w3;
+    }
+  }
+  return best;
 }
 
-// sv_subalgorithm (GJK_SV.cpp:380-410): by the number of points of the
-// simplex, each pattern of slots with its compile-time indices
+// the point of slot k, selected among the values
+DEM_HD d3 sv_slot(const d3 (&y)[4], int k) { return pick(k == 0, y[0], pick(k == 1, y[1], pick(k == 2, y[2], y[3]))); }
+// weight w into slot k of lam
+DEM_HD void sv_put(L4& lam, int k, double w) {
+  lam.w0 = k == 0 ? w : lam.w0;
+  lam.w1 = k == 1 ? w : lam.w1;
+  lam.w2 = k == 2 ? w : lam.w2;
+  lam.w3 = k == 3 ? w : lam.w3;
+}
+
+// sv_subalgorithm (GJK_SV.cpp:380-410) by the number of points of the
+// simplex, its points taken in slot order
 DEM_HD void sv_sub(const d3 (&y)[4], unsigned& bits, L4& lam) {
-  switch (bits) {
-    case 1u: lam.w0 = 1.; break;
-    case 2u: lam.w1 = 1.; break;
-    case 4u: lam.w2 = 1.; break;
-    case 8u: lam.w3 = 1.; break;
-    case 3u: sv_s1d<0, 1>(y, bits, lam); break;
-    case 5u: sv_s1d<0, 2>(y, bits, lam); break;
-    case 6u: sv_s1d<1, 2>(y, bits, lam); break;
-    case 9u: sv_s1d<0, 3>(y, bits, lam); break;
-    case 10u: sv_s1d<1, 3>(y, bits, lam); break;
-    case 12u: sv_s1d<2, 3>(y, bits, lam); break;
-    case 7u: sv_s2d<0, 1, 2>(y, bits, lam); break;
-    case 11u: sv_s2d<0, 1, 3>(y, bits, lam); break;
-    case 13u: sv_s2d<0, 2, 3>(y, bits, lam); break;
-    case 14u: sv_s2d<1, 2, 3>(y, bits, lam); break;
-    default: sv_s3d(y, bits, lam); break;
+  const int n = __builtin_popcount(bits);
+  if (n == 1) {
+    sv_put(lam, __builtin_ctz(bits), 1.);
+  } else if (n == 2) {
+    const int k0 = __builtin_ctz(bits), k1 = 31 - __builtin_clz(bits);
+    double w0, w1;
+    const unsigned k = sv_edge(sv_slot(y, k0), sv_slot(y, k1), w0, w1);
+    sv_put(lam, k0, w0);
+    sv_put(lam, k1, w1);
+    bits = ((k & 1u) ? 1u << k0 : 0u) | ((k & 2u) ? 1u << k1 : 0u);
+  } else if (n == 3) {
+    const int m = __builtin_ctz(~bits & 15u);
+    const int q0 = m == 0 ? 1 : 0, q1 = m <= 1 ? 2 : 1, q2 = m <= 2 ? 3 : 2;
+    double w0, w1, w2;
+    const unsigned k = sv_tri(sv_slot(y, q0), sv_slot(y, q1), sv_slot(y, q2), w0, w1, w2);
+    sv_put(lam, q0, w0);
+    sv_put(lam, q1, w1);
+    sv_put(lam, q2, w2);
+    bits = ((k & 1u) ? 1u << q0 : 0u) | ((k & 2u) ? 1u << q1 : 0u) | ((k & 4u) ? 1u << q2 : 0u);
+  } else {
+    bits = sv_tet(y, lam);
   }
 }
 
-// One iteration of closest_points_GJK_SV with the new point in slot LAST
-// (a compile-time index, as in gjk_iter): 1 to go on, 0 when the loop ends,
-// 2 when the pair is proven farther apart than sep.
+// The search of one iteration of closest_points_GJK_SV with the new point
+// in slot LAST (a compile-time index, as in gjk_iter): 1 when the point
+// joins the simplex (y[LAST] set, bits updated), 0 when the loop ends, 2
+// when the pair is proven farther apart than sep.
 template <int LAST>
 DEM_HD int sv_iter(SvSimplex& S, const ConvexGeo& A, const Tf& a2w, const ConvexGeo& B, const Tf& b2w,
-                   unsigned& bits, d3& v, double& dist, double sep, int it, d3& d, d3& w, bool& accel) {
+                   unsigned& bits, const d3& v, double dist, double sep, int it, d3& d, d3& w, bool& accel) {
   const int go = gjk_search(A, a2w, B, b2w, A.opt, it, v, dist, sep, d, w, accel, S.p[LAST], S.q[LAST]);
   if (go != 1) return go;
   bool degen = false;
@@ -567,9 +589,6 @@ DEM_HD int sv_iter(SvSimplex& S, const ConvexGeo& A, const Tf& a2w, const Convex
   if (degen) return 0;
   S.y[LAST] = w;
   bits |= 1u << LAST;
-  sv_sub(S.y, bits, S.lam);
-  v = sv_point(S.y, bits, S.lam);
-  dist = norm(v);
   return 1;
 }
 
@@ -592,6 +611,9 @@ DEM_HD double gjk_sv_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo&
   int it = 0;
   while (bits < 15 && dist > kEps2 && it < 1000) {
     ++it;
+    const bool plain = !accel;
+    const unsigned bits0 = bits;
+    const d3 v0 = v;
     int go;
     if (!(bits & 1u)) go = sv_iter<0>(S, A, a2w, B, b2w, bits, v, dist, sep, it, d, w, accel);
     else if (!(bits & 2u)) go = sv_iter<1>(S, A, a2w, B, b2w, bits, v, dist, sep, it, d, w, accel);
@@ -603,6 +625,19 @@ DEM_HD double gjk_sv_closest(const ConvexGeo& A, const Tf& a2w, const ConvexGeo&
       return dist;
     }
     if (!go) break;
+    // one copy of the sub-distance for every slot the point went to
+    sv_sub(S.y, bits, S.lam);
+    v = sv_point(S.y, bits, S.lam);
+    dist = norm(v);
+    // A step searched along v that drops its new point again and leaves v
+    // bit for bit as it was is a fixed point: the next step sees the same
+    // simplex and direction, so every step up to the cap repeats it, weights
+    // included.  The loop ends here as it would at the cap (a body against
+    // a long thin box meets this within ten steps).
+    if (plain && bits == bits0 && same_bits(v, v0)) {
+      it = 1000;
+      break;
+    }
   }
   pa = d3{0., 0., 0.};
   pb = d3{0., 0., 0.};

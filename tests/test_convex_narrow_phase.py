@@ -121,3 +121,42 @@ def test_collision_options_equal_oracle(opts):
                 if so in (1, 2):
                     assert np.array_equal(Po, Pg) and np.array_equal(ovo, ovg) and do == dg, (ka, kb, k)
     assert status.get(1, 0) > 150 and status.get(0, 0) > 100
+
+
+@pytest.mark.parametrize("accel", [False, True])
+def test_signed_volume_fixed_point_against_thin_wall(accel):
+    """GJK_SV between a body and a long thin box (a silo wall 2 mm x 0.1 m x
+    1.03 m) can settle on a step that re-adds and drops the same support
+    point, which the loop repeats up to its cap of 1000 steps; the device
+    ends such a loop as soon as a step leaves the simplex and v unchanged.
+    Bodies near the wall at random poses: the oracle reaches the cap on some
+    of them, and both sides give the same bits on all."""
+    from pacific_amd import workloads
+    wl = workloads.silo(200000)
+    Lb, c, _, cr, _ = wl["boxes"][3]
+    o = Oracle((0, 0, 0), (1, 1, 1), (0, 0, 0), 1.0, (0, 0, 0), 0.0, 1e-5, 0)
+    g = GrainsGPU((0, 0, 0), (1, 1, 1))
+    ids = []
+    for s in wl["shapes"]:
+        kw = dict(L=s.get("L"), verts=s.get("verts"), faces=s.get("faces"), crust=s["crust"])
+        ids.append(o.add_shape(s["kind"], **kw))
+        assert g.add_shape(s["kind"], **kw) == ids[-1]
+    wall = o.add_shape(1, L=Lb, crust=cr)
+    assert g.add_shape(1, L=Lb, crust=cr) == wall
+    o.collision(gjk_sv=True, acceleration=accel)
+    g.collision(gjk_sv=True, acceleration=accel)
+    rng = np.random.default_rng(3)
+    eye = np.eye(3).reshape(9)
+    capped, status = 0, {}
+    for k in range(2000):
+        sh = k % len(ids)
+        B = shapes.random_rotations(1, 900 + k)[0]
+        x = np.array([c[0] + Lb[0] / 2 + rng.uniform(0.6e-3, 1.3e-3), c[1] + rng.uniform(-0.04, 0.04),
+                      rng.uniform(0.001, 0.02)])
+        Po, ovo, do, so = o.closest_point(ids[sh], x, B, wall, c, eye)
+        capped += o.gjk_last_iterations() == 1000
+        Pg, ovg, dg, sg = g.closest_point(ids[sh], x, B, wall, c, eye)
+        assert so == sg, (k, so, sg)
+        status[so] = status.get(so, 0) + 1
+        assert np.array_equal(Po, Pg) and np.array_equal(ovo, ovg) and do == dg, k
+    assert capped >= 20 and status.get(1, 0) > 100, (capped, status)
