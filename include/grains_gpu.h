@@ -73,6 +73,23 @@ int grains_gpu_create(const grains_gpu_config* cfg, grains_gpu_system** out);
  * initial time; the system is initialised (grains_gpu_initialize) on return.
  * Replaces GrainsBuilderFactory::create + Grains::do_before_time_stepping. */
 int grains_gpu_create_from_xml(const char* xml_path, int precision, grains_gpu_system** out);
+/* The same input read up to initialisation: the construction data (the
+ * particles inserted at the initial time) without the device, for
+ * inspection with grains_gpu_construction_state.  No counterpart in the
+ * reference, whose Grains::Construction always goes on to the run; a
+ * <Reload> is not read. */
+int grains_gpu_parse_xml(const char* xml_path, int precision, grains_gpu_system** out);
+/* Construction data of a system (id order, before or after initialisation:
+ * the values it was built from): x[3n], v[3n], w[3n], basis[9n] (NULL to
+ * skip; basis only for systems with non-spherical particles). */
+int grains_gpu_construction_state(grains_gpu_system* sys, double* x, double* v, double* w, double* basis);
+/* Insertion over time (<ParticleInsertion><Mode Type="OverTime">,
+ * Grains.cpp:322-327, insertParticle :2070-2140): one insertion call of a
+ * time step, made before the step; the new particle (if one went in) gets
+ * the next id.  grains_gpu_run makes these calls itself.  The number of
+ * particles still waiting: */
+int grains_gpu_try_insert(grains_gpu_system* sys, int* inserted);
+int grains_gpu_pending_insertions(grains_gpu_system* sys, long* n);
 
 void grains_gpu_destroy(grains_gpu_system* sys);
 

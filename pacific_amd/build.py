@@ -19,7 +19,8 @@ EXAMPLE = os.path.join(HERE, "fs_plugin_example")
 
 SOURCES = ["dem_kernels.hip", "dem_forces_sph64.hip", "dem_forces_sph32.hip", "dem_forces_shp64.hip",
            "dem_forces_shp32.hip", "dem_narrow.hip", "dem_noalloc.hip", "dem_batch.hip", "dem_adjoint.hip", "dem_system.cpp", "dem_shapes.cpp",
-           "grains_xml.cpp", "grains_basilisk.cpp", "grains_output.cpp", "grains_reload.cpp"]
+           "grains_xml.cpp", "grains_basilisk.cpp", "grains_output.cpp", "grains_reload.cpp",
+           "grains_insert.cpp"]
 ARCH = os.environ.get("PACIFIC_AMD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -621,6 +621,59 @@ int grains_gpu_system::initialize() {
   return check_status();
 }
 
+int grains_gpu_system::grow(const std::function<int()>& add) {
+  if (has_memory())
+    return fail(this, GRAINS_CONFIG_ERROR, "insertion over time is not supported with a contact model with memory");
+  if (cfg.scheme == GRAINS_ADAMS_BASHFORTH2)
+    return fail(this, GRAINS_CONFIG_ERROR, "insertion over time is not supported with SecondOrderAdamsBashforth");
+  const size_t n = (size_t)nparticles;
+  std::vector<double> F(3 * n), T(3 * n);
+  h_x.resize(3 * n);
+  h_v.resize(3 * n);
+  h_w.resize(3 * n);
+  h_wb.resize(3 * n);
+  h_q.resize(4 * n);
+  int e = grains_gpu_get_state(this, h_x.data(), h_v.data(), h_w.data(), h_wb.data(), h_q.data(), F.data(), T.data());
+  if (e) return e;
+  if (has_shapes()) {
+    h_B.resize(9 * n);
+    if ((e = grains_gpu_get_basis(this, h_B.data()))) return e;
+  }
+  double stat[8];
+  DevStatus st;
+  HIPCHECK(this, hipMemcpy(stat, d_stat_tot, sizeof(stat), hipMemcpyDeviceToHost));
+  HIPCHECK(this, hipMemcpy(&st, d_status, sizeof(st), hipMemcpyDeviceToHost));
+  const double ft = force_time;
+  free_adjoint();
+  free_device();
+  d_hyd = d_fsum = fsum_buf = nullptr;
+  fsum_on = false;
+  initialized = false;
+  needs_rebin = false;
+  cur = 0;
+  if ((e = add())) return e;
+  if ((e = initialize())) return e;
+  // the carried forces of the particles already in; the new one keeps the
+  // weight of the evaluation at initialisation (it touches nothing)
+  const size_t m = (size_t)nparticles;
+  std::vector<int> ids(m);
+  HIPCHECK(this, hipMemcpy(ids.data(), P.id, sizeof(int) * m, hipMemcpyDeviceToHost));
+  std::vector<double> buf(m);
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 2; ++k) {
+      double* dev = k ? P.T[a] : P.F[a];
+      const std::vector<double>& old = k ? T : F;
+      HIPCHECK(this, hipMemcpy(buf.data(), dev, sizeof(double) * m, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < m; ++i)
+        if ((size_t)(ids[i] - 1) < n) buf[i] = old[3 * (size_t)(ids[i] - 1) + a];
+      HIPCHECK(this, hipMemcpy(dev, buf.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    }
+  HIPCHECK(this, hipMemcpy(d_stat_tot, stat, sizeof(stat), hipMemcpyHostToDevice));
+  HIPCHECK(this, hipMemcpy(d_status, &st, sizeof(st), hipMemcpyHostToDevice));
+  force_time = ft;
+  return GRAINS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 
@@ -1213,6 +1266,19 @@ int grains_gpu_set_state(grains_gpu_system* s, const double* x, const double* v,
     if ((e = s->forces(kModeForcesOnly, 0, false))) return e;
   }
   return s->check_status();
+}
+
+int grains_gpu_construction_state(grains_gpu_system* s, double* x, double* v, double* w, double* basis) {
+  if (!s) return GRAINS_ARGUMENT_ERROR;
+  const size_t n = (size_t)s->nparticles;
+  if (x) std::copy(s->h_x.begin(), s->h_x.begin() + 3 * n, x);
+  if (v) std::copy(s->h_v.begin(), s->h_v.begin() + 3 * n, v);
+  if (w) std::copy(s->h_w.begin(), s->h_w.begin() + 3 * n, w);
+  if (basis) {
+    if (s->h_B.size() < 9 * n) return fail(s, GRAINS_ARGUMENT_ERROR, "the system has no non-spherical particles");
+    std::copy(s->h_B.begin(), s->h_B.begin() + 9 * n, basis);
+  }
+  return GRAINS_OK;
 }
 
 int grains_gpu_get_masses(grains_gpu_system* s, double* mass, double* inertia, double* weight) {

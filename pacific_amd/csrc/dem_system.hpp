@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -10,6 +12,9 @@
 
 namespace dem {
 struct AdjointData;
+}
+namespace grains_insert {
+struct Plan;
 }
 
 struct grains_gpu_system {
@@ -182,4 +187,12 @@ struct grains_gpu_system {
   void drop_graphs();
   int recover();
   int check_status();
+  // particles of the XML input waiting for insertion over time
+  // (<Mode Type="OverTime">), tried once per step by the time loop
+  std::shared_ptr<grains_insert::Plan> insertion;
+  // Adds one particle to an initialised system: the state goes back to the
+  // construction data, `add` appends the particle, the device state is built
+  // again from them, and the forces and torques carried to the next step's
+  // first half kick are put back
+  int grow(const std::function<int()>& add);
 };

@@ -232,8 +232,9 @@ int grains_gpu_run(grains_gpu_system* s, int verbose) {
   const double dt = s->cfg.dt, tend = s->t_end;
   const bool raw = !s->rawdata_root.empty();
   const bool vit = !s->save_root.empty();
-  const size_t n = (size_t)s->nparticles;
-  std::vector<int> coord(n, 0);
+  std::vector<int> coord((size_t)s->nparticles, 0);
+  long waiting = 0;
+  auto pending = [&]() { return grains_gpu_pending_insertions(s, &waiting) == GRAINS_OK && waiting > 0; };
   FILE* fv = nullptr;
   int e = GRAINS_OK;
   double vmax = 0., vmean = 0.;
@@ -278,10 +279,26 @@ int grains_gpu_run(grains_gpu_system* s, int verbose) {
     }
     // the contact records of the step that ends at an output time give the
     // coordination numbers; the other steps run without records
+    // while particles wait for insertion over time, one step at a time with
+    // the insertion call before it (Grains.cpp:322-327)
     e = grains_gpu_set_record_pairs(s, rec0);
-    for (long left = k - 1; !e && left > 0; left -= kRunChunk) e = grains_gpu_step(s, (int)std::min(left, kRunChunk));
+    long left = k - 1;
+    int in = 0;
+    while (!e && left > 0) {
+      if (pending()) {
+        e = grains_gpu_try_insert(s, &in);
+        if (!e) e = grains_gpu_step(s, 1);
+        --left;
+      } else {
+        const long c = std::min(left, kRunChunk);
+        e = grains_gpu_step(s, (int)c);
+        left -= c;
+      }
+    }
     if (!e && out && raw) e = grains_gpu_set_record_pairs(s, 1);
+    if (!e && pending()) e = grains_gpu_try_insert(s, &in);
     if (!e) e = grains_gpu_step(s, 1);
+    coord.assign((size_t)s->nparticles, 0);
     if (!e && out && raw) e = grains_gpu_coordination_numbers(s, coord.data());
     grains_gpu_set_record_pairs(s, rec0);
     if (e) break;

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/grains_gpu.h"
+#include "grains_insert.hpp"
 #include "grains_xml.hpp"
 #include "dem_system.hpp"
 
@@ -344,6 +345,12 @@ extern "C" int grains_gpu_create_from_xml(const char* xml_path, int precision, g
   return dem::create_from_xml(xml_path, precision, dem::XmlOptions{}, out);
 }
 
+extern "C" int grains_gpu_parse_xml(const char* xml_path, int precision, grains_gpu_system** out) {
+  dem::XmlOptions opt;
+  opt.no_init = true;
+  return dem::create_from_xml(xml_path, precision, opt, out);
+}
+
 int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& opt, grains_gpu_system** out) {
   if (!xml_path || !out) return GRAINS_ARGUMENT_ERROR;
   *out = nullptr;
@@ -392,9 +399,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
   };
   std::vector<ModelDef> models;
   std::vector<double> positions;
-  double vconst[3] = {0, 0, 0}, wconst[3] = {0, 0, 0};
-  int vmode = 0;  // 0 zero, 1 constant, 2 random
-  double amp_t = 0., amp_r = 0.;
+  grains_insert::Plan plan;
   unsigned seed = 0;
   double t_end = 0.;
   std::vector<double> save_times;
@@ -656,19 +661,29 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
       for (auto& sc : classes) nnew += (size_t)sc.number;
       if (nnew && !ins) c.bad("particles declared without <ParticleInsertion>");
       if (ins && c.ok) {
-        const XNode* mode = ins->child("Mode");
-        if (!mode || mode->attr("Type") != "InitialTime")
-          c.bad("only <Mode Type=\"InitialTime\"> insertion is supported");
-        const XNode* order = ins->child("Order");
-        if (order && order->attr("Type") == "Random") c.bad("only ordered insertion is supported");
-        const XNode* iap = ins->child("InitialAngularPosition");
-        if (iap && (iap->attr("Type") == "Random" || iap->attr("Type") == "File"))
-          c.bad("only fixed initial angular positions are supported");
+        // Grains.cpp:1281-1400
+        if (const XNode* mode = ins->child("Mode")) {
+          std::string t = mode->attr("Type");
+          if (t == "OverTime") plan.over_time = true;
+          else if (t != "InitialTime") c.bad("<Mode Type> is InitialTime or OverTime");
+        }
+        if (const XNode* order = ins->child("Order")) plan.random_order = order->attr("Type") == "Random";
+        if (const XNode* iap = ins->child("InitialAngularPosition")) {
+          if (iap->attr("Type") == "File") c.bad("initial angular positions from a file are not supported");
+          plan.random_angle = iap->attr("Type") == "Random";
+        }
         if (const XNode* rgs = ins->child("RandomGeneratorSeed")) {
           std::string t = rgs->attr("Type");
           if (t == "UserDefined") seed = (unsigned)std::atoi(rgs->attr("Value").c_str());
           else if (t == "Random") c.bad("RandomGeneratorSeed Type=\"Random\" is not reproducible");
         }
+        if (const XNode* fr = ins->child("Frequency")) {
+          long k = std::atol(fr->attr("TryEvery").c_str());
+          if (k < 1) c.bad("<Frequency TryEvery> must be at least 1");
+          plan.frequency = (size_t)std::max(k, 1L);
+        }
+        if (const XNode* fi = ins->child("ForceInsertion")) plan.force = fi->attr("Value") == "True";
+        if (const XNode* go = ins->child("GeometricOverlap")) plan.bv_only = go->attr("Value") == "BV";
         const XNode* pp = ins->child("ParticlePosition");
         if (!pp) c.bad("missing <ParticlePosition>");
         else if (const XNode* fl = pp->child("File")) {
@@ -702,28 +717,78 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
                   positions.push_back(A[2] + (double(m) + 0.5) * dz);
                 }
           }
+        } else if (const XNode* wins = pp->child("Windows")) {
+          // Window::readWindow (Window.cpp:53-230)
+          for (const XNode& wn : wins->kids) {
+            grains_insert::Window w;
+            std::string t = wn.has("Type") ? wn.attr("Type") : std::string("Box");
+            auto point = [&](const XNode* n, double* P) {
+              P[0] = dattr(n, "X", c);
+              P[1] = dattr(n, "Y", c);
+              P[2] = dattr(n, "Z", c);
+            };
+            auto axis = [&](const std::string& d) {
+              if (d == "X") return 0;
+              if (d == "Y") return 1;
+              if (d != "Z") c.bad("window axis Direction is X, Y or Z");
+              return 2;
+            };
+            if (t == "Box" || t == "Line") {
+              w.type = t == "Box" ? grains_insert::kBox : grains_insert::kLine;
+              if (wn.kids.size() < 2) {
+                c.bad("a <Window Type=\"" + t + "\"> needs two points");
+                break;
+              }
+              point(&wn.kids[0], w.A);
+              point(&wn.kids[1], w.B);
+            } else if (t == "Cylinder" || t == "Annulus") {
+              const XNode* bc = wn.child("BottomCentre");
+              const XNode* g = wn.child(t.c_str());
+              if (!bc || !g) {
+                c.bad("a <Window Type=\"" + t + "\"> needs <BottomCentre> and <" + t + ">");
+                break;
+              }
+              point(bc, w.A);
+              if (t == "Cylinder") {
+                w.type = grains_insert::kCylinder;
+                w.radius = dattr(g, "Radius", c);
+              } else {
+                w.type = grains_insert::kAnnulus;
+                w.radius = dattr(g, "RadiusExt", c);
+                w.radius_int = dattr(g, "RadiusInt", c);
+              }
+              w.height = dattr(g, "Height", c);
+              w.axis = axis(g->attr("Direction"));
+            } else {
+              c.bad("unknown insertion window type " + t);
+              break;
+            }
+            plan.windows.push_back(w);
+          }
+          if (plan.windows.empty()) c.bad("<Windows> holds no window");
         } else {
-          c.bad("<ParticlePosition> needs <File> or <StructuredArray>");
+          c.bad("<ParticlePosition> needs <File>, <StructuredArray> or <Windows>");
         }
-        if (c.ok && positions.size() / 3 < nnew) c.bad("fewer positions than particles to insert");
+        if (c.ok && plan.windows.empty() && positions.size() / 3 < nnew)
+          c.bad("fewer positions than particles to insert");
         if (const XNode* iv = ins->child("InitialVelocity")) {
           std::string m = iv->attr("Mode");
           if (m == "Constant") {
-            vmode = 1;
+            plan.vmode = 1;
             if (const XNode* t = iv->child("TranslationalVelocity")) {
-              vconst[0] = dattr(t, "VX", c);
-              vconst[1] = dattr(t, "VY", c);
-              vconst[2] = dattr(t, "VZ", c);
+              plan.vconst[0] = dattr(t, "VX", c);
+              plan.vconst[1] = dattr(t, "VY", c);
+              plan.vconst[2] = dattr(t, "VZ", c);
             }
             if (const XNode* r = iv->child("AngularVelocity")) {
-              wconst[0] = dattr(r, "RX", c);
-              wconst[1] = dattr(r, "RY", c);
-              wconst[2] = dattr(r, "RZ", c);
+              plan.wconst[0] = dattr(r, "RX", c);
+              plan.wconst[1] = dattr(r, "RY", c);
+              plan.wconst[2] = dattr(r, "RZ", c);
             }
           } else if (m == "Random") {
-            vmode = 2;
-            if (const XNode* t = iv->child("Translational")) amp_t = dattr(t, "Amplitude", c);
-            if (const XNode* r = iv->child("Angular")) amp_r = dattr(r, "Amplitude", c);
+            plan.vmode = 2;
+            if (const XNode* t = iv->child("Translational")) plan.amp_t = dattr(t, "Amplitude", c);
+            if (const XNode* r = iv->child("Angular")) plan.amp_r = dattr(r, "Amplitude", c);
           }
         }
       }
@@ -775,38 +840,47 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
                              &shape_of[i]);
     if (e) return e;
   }
-  srand(seed);
-  size_t k = 0;
-  for (auto& sc : classes) {
-    // RawData particle type: the class's index among <Particle> entries
-    s->next_class = (int)(&sc - &classes[0]);
-    for (int i = 0; i < sc.number; ++i, ++k) {
-      double v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
-      if (vmode == 1) {
-        for (int a = 0; a < 3; ++a) { v[a] = vconst[a]; w[a] = wconst[a]; }
-      } else if (vmode == 2) {  // Grains::computeInitVit IV_RANDOM (Grains.cpp:2170-2200)
-        if (amp_t != 0.)
-          for (int a = 0; a < 3; ++a) v[a] = amp_t * (2. * (double(random()) / double(INT_MAX)) - 1.);
-        if (amp_r != 0.) {
-          w[2] = amp_r * (2. * (double(random()) / double(INT_MAX)) - 1.);
-          w[0] = amp_r * (2. * (double(random()) / double(INT_MAX)) - 1.);
-          w[1] = amp_r * (2. * (double(random()) / double(INT_MAX)) - 1.);
-        }
-      }
-      int mat = sc.mat;
-      if (sc.type == dem::kShapeSphere && shape_of[&sc - &classes[0]] < 0)
-        e = grains_gpu_add_spheres(s, 1, &sc.radius, &sc.density, &sc.crust, &mat, &positions[3 * k], v, w);
-      else
-        e = grains_gpu_add_particles(s, shape_of[&sc - &classes[0]], 1, &sc.density, &mat, &positions[3 * k], v,
-                                     w, sc.B);
-      if (e) return e;
-    }
-  }
   for (auto& b : boxes) {
     e = grains_gpu_add_box(s, b.L, b.centre, b.B, b.crust, b.mat);
     if (e) return e;
     s->obs_to_fluid.back() = b.to_fluid;
     s->obs_names.push_back(b.name);
+  }
+  // the particles of the input, inserted at the initial time or over time
+  // (Grains.cpp:2004-2017, 322-327), class by class
+  srand(seed);
+  for (size_t i = 0; i < classes.size(); ++i) {
+    const ParticleClass& sc = classes[i];
+    grains_insert::Class k;
+    k.shape = shape_of[i];
+    k.xml_class = (int)i;
+    k.radius = sc.radius;
+    k.density = sc.density;
+    k.crust = sc.crust;
+    k.mat = sc.mat;
+    for (int j = 0; j < 9; ++j) k.B[j] = sc.B[j];
+    k.remaining = sc.number;
+    plan.classes.push_back(k);
+  }
+  // ordered insertion reverses the list and takes the last entry
+  // (Grains.cpp:1991-1995)
+  for (size_t i = 0; !plan.random_order && i < positions.size() / 2 / 3; ++i)
+    for (int a = 0; a < 3; ++a) std::swap(positions[3 * i + a], positions[positions.size() - 3 * (i + 1) + a]);
+  plan.positions = positions;
+  auto ins_plan = std::make_shared<grains_insert::Plan>(plan);
+  if (!plan.over_time) {
+    if ((e = grains_insert::insert_all(s, *ins_plan, 1000000))) return e;
+  } else {
+    s->insertion = ins_plan;
+    // an initialisation needs a particle: the first steps' attempts are made
+    // here while the system is empty (each one that fails is a step of an
+    // empty system)
+    while (s->nparticles == 0 && ins_plan->pending() > 0) {
+      bool in = false;
+      if ((e = grains_insert::attempt(s, *ins_plan, &in))) return e;
+      if (in) ins_plan->preattempted = true;
+      else s->time += cfg.dt;
+    }
   }
   s->mat_names.assign(dem::kMaxMaterials, std::string());
   for (auto& kv : mats.index) s->mat_names[(size_t)kv.second] = kv.first;
@@ -815,6 +889,7 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
       if (s->shape_files.size() <= (size_t)shape_of[i]) s->shape_files.resize((size_t)shape_of[i] + 1);
       s->shape_files[(size_t)shape_of[i]] = classes[i].file;
     }
+  if (opt.no_init) return GRAINS_OK;
   if ((e = grains_gpu_initialize(s))) return e;
   if (reload_type.empty()) return GRAINS_OK;
   std::string file = reload_file;

@@ -9,6 +9,7 @@ struct XmlOptions {
   double fluid_density = 0.;     // fluid density of a coupled run
   bool track_basis = false;      // keep every particle's Transform basis on the device
   bool fluid_corrected = true;   // (1 - rho_f / rho) acceleration correction
+  bool no_init = false;          // stop before initialisation (the construction data only)
 };
 
 int create_from_xml(const char* xml_path, int precision, const XmlOptions& opt, grains_gpu_system** out);

@@ -52,7 +52,8 @@ EXPORTS = [
     "grains_gpu_graph_stats", "grains_gpu_force_sums", "grains_gpu_set_hydro_ft",
     "grains_gpu_coordination_at_state", "grains_gpu_post_processing", "grains_gpu_save_reload",
     "grains_gpu_load_reload", "grains_gpu_reload_same", "grains_gpu_force_sums_noalloc",
-    "grains_gpu_set_collision",
+    "grains_gpu_set_collision", "grains_gpu_parse_xml", "grains_gpu_construction_state", "grains_gpu_try_insert",
+    "grains_gpu_pending_insertions",
 ]
 
 
@@ -102,6 +103,9 @@ def lib():
         L.grains_gpu_write_rawdata.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ip]
         L.grains_gpu_set_outputs.argtypes = [vp, dp, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.grains_gpu_run.argtypes = [vp, ctypes.c_int]
+        L.grains_gpu_construction_state.argtypes = [vp, dp, dp, dp, dp]
+        L.grains_gpu_try_insert.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.grains_gpu_pending_insertions.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
         L.grains_gpu_graph_stats.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_long), ctypes.c_char_p, sz]
         L.grains_gpu_get_pairs.argtypes = [vp, ip, sz, ctypes.POINTER(sz)]
         L.grains_gpu_num_particles.argtypes = [vp]
@@ -270,9 +274,12 @@ class GrainsGPU:
         return a.value, b.value
 
     @classmethod
-    def from_xml(cls, path, precision=FP64):
+    def from_xml(cls, path, precision=FP64, init=True):
+        """A system built from a Grains3D input file; init=False stops before
+        initialisation (no device needed): the construction data only."""
         h = ctypes.c_void_p()
-        code = lib().grains_gpu_create_from_xml(os.fsencode(path), precision, ctypes.byref(h))
+        make = lib().grains_gpu_create_from_xml if init else lib().grains_gpu_parse_xml
+        code = make(os.fsencode(path), precision, ctypes.byref(h))
         if code:
             msg = lib().grains_gpu_last_error(h).decode() if h.value else "create failed"
             if h.value:
@@ -535,9 +542,34 @@ class GrainsGPU:
                                                  os.fsencode(rawdata_root) if rawdata_root else None,
                                                  int(bool(rawdata_binary))))
 
+    def construction_state(self):
+        """The values the system was built from (id order): dict(x, v, w[, B])."""
+        n = self.n = int(lib().grains_gpu_num_particles(self.h))
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "w")}
+        B = np.zeros((n, 9))
+        if lib().grains_gpu_construction_state(self.h, _d(out["x"]), _d(out["v"]), _d(out["w"]), _d(B)) == 0:
+            out["B"] = B.reshape(n, 3, 3)
+        else:
+            self._check(lib().grains_gpu_construction_state(self.h, _d(out["x"]), _d(out["v"]), _d(out["w"]), None))
+        return out
+
+    def try_insert(self):
+        """One insertion call of a time step (insertion over time): True if a
+        particle went in."""
+        k = ctypes.c_int()
+        self._check(lib().grains_gpu_try_insert(self.h, ctypes.byref(k)))
+        self.n = int(lib().grains_gpu_num_particles(self.h))
+        return bool(k.value)
+
+    def pending_insertions(self):
+        k = ctypes.c_long()
+        self._check(lib().grains_gpu_pending_insertions(self.h, ctypes.byref(k)))
+        return k.value
+
     def run(self, verbose=False):
         """Grains::Simulation to the end time with its outputs."""
         self._check(lib().grains_gpu_run(self.h, int(bool(verbose))))
+        self.n = int(lib().grains_gpu_num_particles(self.h))
 
     def contact_counts(self):
         """(directed contact records, of which on the Coulomb cap) of the last

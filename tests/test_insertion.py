@@ -1,0 +1,184 @@
+"""Particle insertion of the XML front-end (Grains::insertParticle,
+Grains.cpp:2030-2200; Window::getInsertionPoint, Window.cpp:238-320;
+GrainsExec::RandomRotationMatrix, GrainsExec.cpp:476-502): random points of
+insertion windows, random initial velocities and angular positions, and the
+overlap test of LinkedCell::insertParticleSerial.  The expected values are
+restated here from the C library's own random() stream (the same generator
+the library draws from, reseeded by the test), so positions, velocities and
+bases must agree bit for bit.  Read before initialisation
+(grains_gpu_parse_xml): no device needed."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from pacific_amd import GrainsGPU, shapes
+from tests.test_xml import write_case
+
+libc = ctypes.CDLL(None)
+libc.random.restype = ctypes.c_long
+libc.rand.restype = ctypes.c_int
+libc.srand.argtypes = [ctypes.c_uint]
+INT_MAX = 2147483647
+
+
+def ur():
+    return libc.random() / INT_MAX
+
+
+def window_point(w):
+    """Window::getInsertionPoint"""
+    if w[0] == "Box":
+        A, B = w[1], w[2]
+        return [A[a] + ur() * (B[a] - A[a]) for a in range(3)]
+    if w[0] == "Line":
+        t = ur()
+        return [w[1][a] + t * (w[2][a] - w[1][a]) for a in range(3)]
+    kind, C, radius, rint, height = w[0], w[1], w[2], w[3], w[4]
+    r = ur() * radius if kind == "Cylinder" else rint + ur() * (radius - rint)
+    theta = ur() * 2. * math.pi
+    ax = ur() * height
+    return [C[0] + r * math.cos(theta), C[1] + r * math.sin(theta), C[2] + ax]
+
+
+def window_xml(w):
+    if w[0] in ("Box", "Line"):
+        pts = "".join(f'<Point X="{p[0]!r}" Y="{p[1]!r}" Z="{p[2]!r}"/>' for p in (w[1], w[2]))
+        return f'<Window Type="{w[0]}">{pts}</Window>'
+    C = w[1]
+    geo = (f'<Cylinder Radius="{w[2]!r}" Height="{w[4]!r}" Direction="Z"/>' if w[0] == "Cylinder" else
+           f'<Annulus RadiusExt="{w[2]!r}" RadiusInt="{w[3]!r}" Height="{w[4]!r}" Direction="Z"/>')
+    return (f'<Window Type="{w[0]}"><BottomCentre X="{C[0]!r}" Y="{C[1]!r}" Z="{C[2]!r}"/>{geo}</Window>')
+
+
+def insertion_xml(tmp_path, n, windows, seed, extra="", amp=(0.3, 2.0)):
+    ins = (f'<Mode Type="InitialTime"/><Order Type="Ordered"/>{extra}'
+           f'<RandomGeneratorSeed Type="UserDefined" Value="{seed}"/>'
+           f'<ParticlePosition><Windows>{"".join(window_xml(w) for w in windows)}</Windows></ParticlePosition>'
+           f'<InitialVelocity Mode="Random"><Translational Amplitude="{amp[0]!r}"/>'
+           f'<Angular Amplitude="{amp[1]!r}"/></InitialVelocity>')
+    path, wl = write_case(tmp_path, n=n)
+    text = open(path).read()
+    i, j = text.index("<ParticleInsertion>"), text.index("</ParticleInsertion>")
+    text = text[:i] + "<ParticleInsertion>" + ins + text[j:]
+    open(path, "w").write(text)
+    return path
+
+
+def restate_spheres(n, windows, seed, r=1e-3, amp=(0.3, 2.0)):
+    """Insertion of n spheres of radius r: window, point, velocities, then
+    the sphere-sphere test |x_B - x_A| - (r_A + r_B) < 0 against those in."""
+    libc.srand(seed)
+    xs, vs, ws, attempts = [], [], [], 0
+    while len(xs) < n:
+        w = windows[0]
+        if len(windows) != 1:
+            k = int(ur() * len(windows))
+            w = windows[min(k, len(windows) - 1)]
+        P = window_point(w)
+        v = [amp[0] * (2. * ur() - 1.) for _ in range(3)]
+        wz, wx, wy = (amp[1] * (2. * ur() - 1.) for _ in range(3))
+        attempts += 1
+        hit = False
+        for q in xs:
+            dx, dy, dz = P[0] - q[0], P[1] - q[1], P[2] - q[2]
+            if math.sqrt(dx * dx + dy * dy + dz * dz) - (r + r) < 0.:
+                hit = True
+                break
+        if not hit:
+            xs.append(P)
+            vs.append(v)
+            ws.append([wx, wy, wz])
+    return np.array(xs), np.array(vs), np.array(ws), attempts
+
+
+@pytest.mark.parametrize("case", ["one_box", "mixed"])
+def test_window_insertion_equals_restatement(tmp_path, case):
+    if case == "one_box":
+        # 150 spheres in a box window small enough that attempts fail
+        windows = [("Box", (0.003, 0.003, 0.003), (0.017, 0.017, 0.025))]
+        n = 150
+    else:
+        windows = [("Box", (0.003, 0.003, 0.003), (0.017, 0.017, 0.012)),
+                   ("Cylinder", (0.01, 0.01, 0.015), 0.006, 0., 0.01),
+                   ("Annulus", (0.01, 0.01, 0.03), 0.007, 0.003, 0.02),
+                   ("Line", (0.004, 0.004, 0.052), (0.016, 0.016, 0.052))]
+        n = 120
+    path = insertion_xml(tmp_path, n, windows, seed=17)
+    g = GrainsGPU.from_xml(path, init=False)
+    st = g.construction_state()
+    x, v, w, attempts = restate_spheres(n, windows, 17)
+    assert attempts > n + 10  # the overlap test rejected points
+    assert np.array_equal(st["x"], x)
+    assert np.array_equal(st["v"], v)
+    assert np.array_equal(st["w"], w)
+
+
+def restate_rotation():
+    def mul(m, q):
+        return [[m[i][0] * q[0][j] + m[i][1] * q[1][j] + m[i][2] * q[2][j] for j in range(3)] for i in range(3)]
+    az = 2. * math.pi * libc.rand() / INT_MAX
+    rZ = [[math.cos(az), -math.sin(az), 0.], [math.sin(az), math.cos(az), 0.], [0., 0., 1.]]
+    ax = 2. * math.pi * libc.rand() / INT_MAX
+    rX = [[1., 0., 0.], [0., math.cos(ax), -math.sin(ax)], [0., math.sin(ax), math.cos(ax)]]
+    ay = 2. * math.pi * libc.rand() / INT_MAX
+    rY = [[math.cos(ay), 0., math.sin(ay)], [0., 1., 0.], [-math.sin(ay), 0., math.cos(ay)]]
+    return mul(rX, mul(rY, rZ)), mul
+
+
+def test_random_angular_positions_and_random_order(tmp_path):
+    """Two polyhedron classes, random class order, random rotations composed
+    on the class's angular position, ForceInsertion (no overlap test): the
+    class picks, points, rotations and velocities of the stream."""
+    v, f = shapes.tetrahedron(1.2e-3)
+    shapes.write_insert(str(tmp_path / "tet.insert"), v, f)
+    windows = [("Box", (0.003, 0.003, 0.003), (0.017, 0.017, 0.05))]
+    path = insertion_xml(tmp_path, 7, windows, seed=5,
+                         extra='<ForceInsertion Value="True"/><InitialAngularPosition Type="Random"/>',
+                         amp=(0.0, 1.5))
+    text = open(path).read().replace('<Order Type="Ordered"/>', '<Order Type="Random"/>')
+    cls2 = """      <Particle Number="5" Density="2000.">
+        <Convex CrustThickness="5.e-5">
+          <Polyhedron Name="tet.insert"/>
+        </Convex>
+        <AngularPosition Type="Identity"></AngularPosition>
+        <Material>matP</Material>
+      </Particle>
+    </Particles>"""
+    text = text.replace("    </Particles>", cls2, 1)
+    open(path, "w").write(text)
+    g = GrainsGPU.from_xml(path, init=False)
+    st = g.construction_state()
+    libc.srand(5)
+    left = [7, 5]
+    xs, ws, Bs, kinds = [], [], [], []
+    while sum(left):
+        rem = [i for i in range(2) if left[i]]
+        k = rem[min(int(len(rem) * ur()), len(rem) - 1)]
+        P = window_point(windows[0])
+        R, mul = restate_rotation()
+        B = mul([[1., 0., 0.], [0., 1., 0.], [0., 0., 1.]], R)
+        wz, wx, wy = (1.5 * (2. * ur() - 1.) for _ in range(3))
+        left[k] -= 1
+        xs.append(P)
+        ws.append([wx, wy, wz])
+        Bs.append(B)
+        kinds.append(k)
+    assert 0 < sum(kinds) < 12  # both classes drawn
+    assert np.array_equal(st["x"], np.array(xs))
+    assert np.array_equal(st["w"], np.array(ws))
+    assert np.array_equal(st["B"], np.array(Bs))
+
+
+def test_over_time_waits(tmp_path):
+    """<Mode Type="OverTime">: the first particle goes in before the first
+    step, the others wait for the time loop."""
+    windows = [("Box", (0.003, 0.003, 0.003), (0.017, 0.017, 0.05))]
+    path = insertion_xml(tmp_path, 40, windows, seed=3, extra='<Frequency TryEvery="10"/>')
+    text = open(path).read().replace('<Mode Type="InitialTime"/>', '<Mode Type="OverTime"/>')
+    open(path, "w").write(text)
+    g = GrainsGPU.from_xml(path, init=False)
+    assert g.n == 1 and g.pending_insertions() == 39
+    x, _, _, _ = restate_spheres(1, windows, 3)
+    assert np.array_equal(g.construction_state()["x"], x)

@@ -115,7 +115,10 @@ def test_valid_file_parses(tmp_path):
 @pytest.mark.parametrize("sub,frag", [
     (("<Hooke>", "<HookeMassless>", "</Hooke>", "</HookeMassless>"), "not supported"),
     (("<Sphere Radius=\"0.001\"/>", "<Cylinder Radius=\"0.001\" Height=\"0.002\"/>"), "other than <Sphere>, <Box> and <Polyhedron>"),
-    (("<Mode Type=\"InitialTime\"/>", "<Mode Type=\"OverTime\"/>"), "InitialTime"),
+    (("<Mode Type=\"InitialTime\"/>", "<Mode Type=\"Sometimes\"/>"), "InitialTime or OverTime"),
+    (("<File Name=\"positions.result\"/>", "<Windows><Window Type=\"Sphere\"/></Windows>"),
+     "unknown insertion window type"),
+    (("<File Name=\"positions.result\"/>", "<Windows></Windows>"), "holds no window"),
     (("<Gravity GX=\"0.\" GY=\"0.\" GZ=\"-9.81\"/>", ""), "Gravity"),
     (("<TimeStep dt=\"1.e-5\"/>", "<TimeStep/>"), "missing attribute dt"),
     (("</Construction>", "</Construktion>"), "does not match"),
