@@ -155,30 +155,93 @@ bool touching(const ConvexGeo& A, const ConvexGeo& B, bool bv_only) {
   return d < A.crust + B.crust - kEps;
 }
 
-// whether the new body touches a particle in place (the nearest periodic
-// image of each) or an obstacle
-bool blocked(const grains_gpu_system* s, const Plan& p, const ConvexGeo& nb, const std::vector<double>& x,
-             const std::vector<double>& B) {
-  const size_t n = x.size() / 3;
+// cell coordinates of a point (wrapped on periodic axes) and their key
+void cell_of(const grains_gpu_system* s, const Plan& p, const double* x, long* k) {
+  for (int a = 0; a < 3; ++a) {
+    k[a] = (long)std::floor((x[a] - s->cfg.origin[a]) / p.h[a]);
+    if (p.ncell[a] > 0) k[a] = ((k[a] % p.ncell[a]) + p.ncell[a]) % p.ncell[a];
+  }
+}
+long long cell_key(const long* k) {
+  return ((long long)(k[0] & 0x1FFFFF) << 42) | ((long long)(k[1] & 0x1FFFFF) << 21) | (long long)(k[2] & 0x1FFFFF);
+}
+
+// the cell edge: the largest circumscribed diameter among the classes and
+// the particles in place
+void setup_cells(const grains_gpu_system* s, Plan& p) {
+  double d = 0.;
+  for (const Class& c : p.classes) d = std::max(d, 2. * (c.shape < 0 ? c.radius : s->shapes[(size_t)c.shape].circR));
+  for (size_t j = 0; j < (size_t)s->nparticles; ++j) {
+    const int shape = s->h_shape.empty() ? -1 : s->h_shape[j];
+    d = std::max(d, 2. * (shape < 0 ? s->h_rad[j] : s->shapes[(size_t)shape].circR));
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double L = s->cfg.max[a] - s->cfg.origin[a];
+    p.h[a] = d > 0. ? d : 1.;
+    p.ncell[a] = 0;
+    if (s->cfg.periodic[a]) {
+      p.ncell[a] = std::max(1L, (long)std::floor(L / p.h[a]));
+      p.h[a] = L / double(p.ncell[a]);
+    }
+  }
+  p.cells.clear();
+  for (size_t j = 0; j < (size_t)s->nparticles; ++j) {
+    long k[3];
+    cell_of(s, p, &s->h_x[3 * j], k);
+    p.cells[cell_key(k)].push_back((int)j);
+  }
+}
+
+// whether the new body touches particle j (its nearest periodic image)
+bool touches_particle(const grains_gpu_system* s, const Plan& p, const ConvexGeo& nb, const std::vector<double>& x,
+                      const std::vector<double>& B, size_t j) {
   double L[3];
   for (int a = 0; a < 3; ++a) L[a] = s->cfg.periodic[a] ? s->cfg.max[a] - s->cfg.origin[a] : 0.;
   const double c[3] = {nb.o.x, nb.o.y, nb.o.z};
-  for (size_t j = 0; j < n; ++j) {
-    const int shape = s->h_shape.empty() ? -1 : s->h_shape[j];
-    const double reach = nb.circR + (shape < 0 ? s->h_rad[j] : s->shapes[(size_t)shape].circR);
-    double xj[3];
-    bool near = true;
-    for (int a = 0; a < 3 && near; ++a) {
-      const double dx = c[a] - x[3 * j + a];
-      double sh = 0.;
-      if (L[a] > 0. && dx > 0.5 * L[a]) sh = L[a];
-      else if (L[a] > 0. && dx < -0.5 * L[a]) sh = -L[a];
-      xj[a] = x[3 * j + a] + sh;
-      near = std::fabs(dx - sh) < reach;
-    }
-    if (!near) continue;
-    const ConvexGeo A = body_geo(s, shape, s->h_rad[j], s->h_crust[j], xj, B.empty() ? nullptr : &B[9 * j]);
-    if (touching(A, nb, p.bv_only)) return true;
+  const int shape = s->h_shape.empty() ? -1 : s->h_shape[j];
+  const double reach = nb.circR + (shape < 0 ? s->h_rad[j] : s->shapes[(size_t)shape].circR);
+  double xj[3];
+  for (int a = 0; a < 3; ++a) {
+    const double dx = c[a] - x[3 * j + a];
+    double sh = 0.;
+    if (L[a] > 0. && dx > 0.5 * L[a]) sh = L[a];
+    else if (L[a] > 0. && dx < -0.5 * L[a]) sh = -L[a];
+    xj[a] = x[3 * j + a] + sh;
+    if (!(std::fabs(dx - sh) < reach)) return false;
+  }
+  const ConvexGeo A = body_geo(s, shape, s->h_rad[j], s->h_crust[j], xj, B.empty() ? nullptr : &B[9 * j]);
+  return touching(A, nb, p.bv_only);
+}
+
+// whether the new body touches a particle in place or an obstacle: before
+// initialisation through the cells, afterwards over all particles
+bool blocked(const grains_gpu_system* s, const Plan& p, const ConvexGeo& nb, const std::vector<double>& x,
+             const std::vector<double>& B) {
+  if (!s->initialized) {
+    const double c[3] = {nb.o.x, nb.o.y, nb.o.z};
+    long k0[3];
+    cell_of(s, p, c, k0);
+    long long seen[27];
+    int nseen = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          long k[3] = {k0[0] + dx, k0[1] + dy, k0[2] + dz};
+          for (int a = 0; a < 3; ++a)
+            if (p.ncell[a] > 0) k[a] = ((k[a] % p.ncell[a]) + p.ncell[a]) % p.ncell[a];
+          const long long key = cell_key(k);
+          bool dup = false;
+          for (int i = 0; i < nseen; ++i) dup |= seen[i] == key;
+          if (dup) continue;
+          seen[nseen++] = key;
+          auto it = p.cells.find(key);
+          if (it == p.cells.end()) continue;
+          for (int j : it->second)
+            if (touches_particle(s, p, nb, x, B, (size_t)j)) return true;
+        }
+  } else {
+    for (size_t j = 0; j < x.size() / 3; ++j)
+      if (touches_particle(s, p, nb, x, B, j)) return true;
   }
   for (const BoxObstacle& o : s->obstacles)
     if (touching(obstacle_geo(s, o), nb, p.bv_only)) return true;
@@ -260,6 +323,7 @@ int attempt(grains_gpu_system* s, Plan& p, bool* inserted) {
       xs = s->h_x;
       if (s->has_shapes()) Bs = s->h_B;
     }
+    if (!s->initialized && p.h[0] == 0.) setup_cells(s, p);
     const ConvexGeo nb = body_geo(s, c.shape, c.radius, c.crust, P, p.wait_B);
     if (p.force || !blocked(s, p, nb, xs, Bs)) {
       s->next_class = c.xml_class;
@@ -268,8 +332,14 @@ int attempt(grains_gpu_system* s, Plan& p, bool* inserted) {
         if (c.shape < 0) return grains_gpu_add_spheres(s, 1, &c.radius, &c.density, &c.crust, &mat, P, v, w);
         return grains_gpu_add_particles(s, c.shape, 1, &c.density, &mat, P, v, w, p.wait_B);
       };
-      e = s->initialized ? s->grow(add) : add();
+      const bool live = s->initialized;
+      e = live ? s->grow(add) : add();
       if (e) return e;
+      if (!live) {
+        long k[3];
+        cell_of(s, p, P, k);
+        p.cells[cell_key(k)].push_back(s->nparticles - 1);
+      }
       if (npos) {
         if (p.random_order)
           for (int a = 0; a < 3; ++a) p.positions[3 * isp + a] = p.positions[3 * (npos - 1) + a];

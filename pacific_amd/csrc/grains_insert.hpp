@@ -7,6 +7,7 @@
 // construction data, over time it grows an initialised system.
 #pragma once
 #include <cstddef>
+#include <unordered_map>
 #include <vector>
 
 struct grains_gpu_system;
@@ -54,6 +55,13 @@ struct Plan {
   // the attempt of the next step was made before initialisation
   bool preattempted = false;
   long attempts = 0, inserted = 0;
+  // particles in place before initialisation, binned on cells of edge at
+  // least the largest circumscribed diameter (per axis: h[a], n[a] cells
+  // on a periodic axis), so that the overlap test of an initial-time fill
+  // visits 27 cells
+  double h[3] = {0., 0., 0.};
+  long ncell[3] = {0, 0, 0};
+  std::unordered_map<long long, std::vector<int>> cells;
 
   long pending() const {
     long n = 0;

@@ -182,3 +182,42 @@ def test_over_time_waits(tmp_path):
     assert g.n == 1 and g.pending_insertions() == 39
     x, _, _, _ = restate_spheres(1, windows, 3)
     assert np.array_equal(g.construction_state()["x"], x)
+
+
+def test_window_insertion_periodic(tmp_path):
+    """A domain periodic in x and y: the overlap test sees the nearest
+    periodic image of each particle in place (binned on wrapped cells)."""
+    windows = [("Box", (0., 0., 0.003), (0.02, 0.02, 0.012))]
+    n = 120
+    path = insertion_xml(tmp_path, n, windows, seed=23)
+    text = open(path).read().replace('<Periodicity PX="0" PY="0" PZ="0"/>', '<Periodicity PX="1" PY="1" PZ="0"/>')
+    i, j = text.index("<Obstacles>"), text.index("</Obstacles>")
+    text = text[:i] + text[j + len("</Obstacles>"):]  # no walls across the periodic axes
+    open(path, "w").write(text)
+    g = GrainsGPU.from_xml(path, init=False)
+    st = g.construction_state()
+    L, r = 0.02, 1e-3
+    libc.srand(23)
+    xs, wraps = [], 0
+    while len(xs) < n:
+        P = window_point(windows[0])
+        for _ in range(6):
+            ur()
+        hit = False
+        for q in xs:
+            xj = list(q)
+            for a in range(2):
+                dx = P[a] - q[a]
+                if dx > 0.5 * L:
+                    xj[a] = q[a] + L
+                elif dx < -0.5 * L:
+                    xj[a] = q[a] - L
+            d = [P[a] - xj[a] for a in range(3)]
+            if math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - (r + r) < 0.:
+                hit = True
+                wraps += xj != list(q)
+                break
+        if not hit:
+            xs.append(P)
+    assert wraps > 0  # some rejections came from periodic images
+    assert np.array_equal(st["x"], np.array(xs))
