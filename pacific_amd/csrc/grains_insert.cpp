@@ -290,6 +290,8 @@ int attempt(grains_gpu_system* s, Plan& p, bool* inserted) {
       double R[9];
       random_rotation(R);
       mat_mul(p.wait_B, R, p.wait_B);
+    } else if (p.angle_next < p.angles.size() / 9) {
+      mat_mul(p.wait_B, &p.angles[9 * p.angle_next], p.wait_B);
     }
     // Grains::computeInitVit (Grains.cpp:2148-2200)
     double v[3] = {0., 0., 0.}, w[3] = {0., 0., 0.};
@@ -345,6 +347,7 @@ int attempt(grains_gpu_system* s, Plan& p, bool* inserted) {
           for (int a = 0; a < 3; ++a) p.positions[3 * isp + a] = p.positions[3 * (npos - 1) + a];
         p.positions.resize(3 * (npos - 1));
       }
+      if (p.angle_next < p.angles.size() / 9) ++p.angle_next;
       --p.classes[(size_t)p.wait].remaining;
       p.wait = -1;
       ++p.inserted;

@@ -37,6 +37,10 @@ struct Plan {
   bool over_time = false;  // <Mode Type="OverTime">
   bool random_order = false;  // <Order Type="Random">
   bool random_angle = false;  // <InitialAngularPosition Type="Random">
+  // <InitialAngularPosition Type="File">: one matrix (9, by rows) per
+  // particle, taken in file order
+  std::vector<double> angles;
+  size_t angle_next = 0;
   bool force = false;  // <ForceInsertion Value="True">
   bool bv_only = false;  // <GeometricOverlap Value="BV">
   size_t frequency = 1;  // <Frequency TryEvery>

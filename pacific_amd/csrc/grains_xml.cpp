@@ -669,8 +669,19 @@ int dem::create_from_xml(const char* xml_path, int precision, const XmlOptions& 
         }
         if (const XNode* order = ins->child("Order")) plan.random_order = order->attr("Type") == "Random";
         if (const XNode* iap = ins->child("InitialAngularPosition")) {
-          if (iap->attr("Type") == "File") c.bad("initial angular positions from a file are not supported");
           plan.random_angle = iap->attr("Type") == "Random";
+          if (iap->attr("Type") == "File") {
+            // Grains::setAngularPositionParticlesFromFile (Grains.cpp:2256-2300):
+            // one matrix by rows per particle to insert
+            std::string name = iap->attr("Name");
+            std::ifstream af(name);
+            if (!af) af.open(dir + name);
+            if (!af) c.bad("cannot open angular position file " + name);
+            double m;
+            while (af >> m) plan.angles.push_back(m);
+            if (c.ok && plan.angles.size() != 9 * nnew)
+              c.bad("the number of angular positions in " + name + " differs from the number of particles");
+          }
         }
         if (const XNode* rgs = ins->child("RandomGeneratorSeed")) {
           std::string t = rgs->attr("Type");

@@ -221,3 +221,29 @@ def test_window_insertion_periodic(tmp_path):
             xs.append(P)
     assert wraps > 0  # some rejections came from periodic images
     assert np.array_equal(st["x"], np.array(xs))
+
+
+def test_angular_positions_from_file(tmp_path):
+    """<InitialAngularPosition Type="File">: one matrix by rows per particle,
+    composed on the right of the class's angular position, in file order."""
+    v, f = shapes.tetrahedron(1.2e-3)
+    shapes.write_insert(str(tmp_path / "tet.insert"), v, f)
+    windows = [("Box", (0.003, 0.003, 0.003), (0.017, 0.017, 0.05))]
+    path = insertion_xml(tmp_path, 4, windows, seed=9,
+                         extra='<ForceInsertion Value="True"/><InitialAngularPosition Type="File" Name="ang.txt"/>',
+                         amp=(0.0, 0.0))
+    text = open(path).read().replace("<Sphere Radius=\"0.001\"/>", "<Polyhedron Name=\"tet.insert\"/>")
+    open(path, "w").write(text)
+    R = shapes.random_rotations(4, 77).reshape(4, 3, 3)
+    np.savetxt(tmp_path / "ang.txt", R.reshape(4, 9), fmt="%.17g")
+    g = GrainsGPU.from_xml(path, init=False)
+    B = g.construction_state()["B"]
+    eye = [[1., 0., 0.], [0., 1., 0.], [0., 0., 1.]]
+    _, mul = restate_rotation()
+    for k in range(4):
+        assert np.array_equal(B[k], np.array(mul(eye, R[k].tolist())))
+    bad = text.replace('<Particle Number="4"', '<Particle Number="5"')
+    open(path, "w").write(bad)
+    with pytest.raises(Exception) as e:
+        GrainsGPU.from_xml(path, init=False)
+    assert "angular positions" in str(e.value)
