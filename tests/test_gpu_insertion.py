@@ -75,3 +75,33 @@ def test_over_time_in_the_time_loop(tmp_path, gpu):
     g.run()
     # 100 steps: attempts before steps 1, 6, 11, ... 96
     assert g.n == 20 and g.pending_insertions() == 20
+
+
+def test_over_time_growth_polyhedra(tmp_path, gpu):
+    """The same with tetrahedra at random orientations: the bases of the
+    bodies already in carry over the growth bit for bit too."""
+    from pacific_amd import shapes
+    pos = workloads.settling_box(20)["x"] - np.array([0., 0., 0.68e-3])
+    far = np.array([[0.01, 0.01, 0.05]])
+    # one system after the other: both draw their rotations from the C
+    # library's one random stream, reseeded as each file is read
+    systems = []
+    for name, p in (("pa", np.vstack([pos, far])), ("pb", pos)):
+        path = over_time_case(tmp_path, name, p)
+        v, f = shapes.tetrahedron(1.2e-3)
+        shapes.write_insert(str(tmp_path / name / "tet.insert"), v, f)
+        text = open(path).read().replace('<Sphere Radius="0.001"/>', '<Polyhedron Name="tet.insert"/>')
+        text = text.replace('<Mode Type="OverTime"/>', '<Mode Type="OverTime"/><InitialAngularPosition Type="Random"/>')
+        text = text.replace('<RandomGeneratorSeed Type="Default"/>', '<RandomGeneratorSeed Type="UserDefined" Value="4"/>')
+        open(path, "w").write(text)
+        g = GrainsGPU.from_xml(path)
+        for _ in range(300):
+            g.try_insert()
+            g.step(1)
+        systems.append(g)
+    a, b = systems
+    assert a.n == 21 and b.n == 20
+    sa, sb = a.state(), b.state()
+    for k in ("x", "v", "w", "q"):
+        assert np.array_equal(sa[k][:20], sb[k][:20]), k
+    assert np.array_equal(a.basis()[:20], b.basis()[:20])
