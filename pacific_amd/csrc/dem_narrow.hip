@@ -244,13 +244,17 @@ __device__ __forceinline__ void narrow_one(const Particles& P, const Sorted& S, 
 }
 
 // The work list of sorted slot k (see k_items); returns its number of items
-// and, with out != nullptr, writes them from out[0].
+// and, with out != nullptr, writes them from out[0].  With keep != nullptr
+// (out == nullptr) it also returns which candidates (keep[0], bit j) and
+// obstacles (keep[1], bit o) give items and the last obstacle met (keep[2];
+// -2: no body in the slot), for write_items.
 __device__ __forceinline__ int body_items(const Particles& P, const Sorted& S, const BoxObstacle* __restrict__ obs,
                                           const StepParams& sp, const NbrList& L, int k,
-                                          unsigned long long* __restrict__ out) {
+                                          unsigned long long* __restrict__ out, long long* keep = nullptr) {
   const int s = S.order[k];
   if ((unsigned)s >= (unsigned)sp.n) {
     if (out) sp.cf.count[k] = 0;
+    if (keep) keep[2] = -2;
     return 0;
   }
   const int nc = L.count[k];
@@ -272,6 +276,7 @@ __device__ __forceinline__ int body_items(const Particles& P, const Sorted& S, c
       }
     }
     if (out) out[c] = key | (unsigned)j;
+    if (keep && j < 64) keep[0] |= 1ll << j;
     ++c;
   }
   int last = -1;
@@ -280,10 +285,12 @@ __device__ __forceinline__ int body_items(const Particles& P, const Sorted& S, c
     if (fabs(B.aabb_c[0] - x[0]) <= B.aabb_e[0] + r && fabs(B.aabb_c[1] - x[1]) <= B.aabb_e[1] + r &&
         fabs(B.aabb_c[2] - x[2]) <= B.aabb_e[2] + r) {
       if (out) out[c] = key | (unsigned)(nc + o);
+      if (keep && o < 64) keep[1] |= 1ll << o;
       ++c;
       last = o;
     }
   }
+  if (keep) keep[2] = last;
   if (out) {
     // every obstacle slot the force sweep will read, met or not, starts empty
     for (int o = 0; o <= last; ++o) sp.cf.ent[(size_t)(nc + o) * sp.cf.n + k] = kNoFeat;
@@ -292,15 +299,36 @@ __device__ __forceinline__ int body_items(const Particles& P, const Sorted& S, c
   return c;
 }
 
+// The items body_items found for slot k, written from the masks it kept
+// (no second pass over the candidate list), in the same order.
+__device__ __forceinline__ void write_items(const StepParams& sp, int k, int nc, const long long* keep,
+                                            unsigned long long* __restrict__ out) {
+  if (keep[2] == -2) {
+    sp.cf.count[k] = 0;
+    return;
+  }
+  const unsigned long long key = (unsigned long long)(unsigned)k << 32;
+  int c = 0;
+  for (unsigned long long m = (unsigned long long)keep[0]; m; m &= m - 1)
+    out[c++] = key | (unsigned)__builtin_ctzll(m);
+  for (unsigned long long m = (unsigned long long)keep[1]; m; m &= m - 1)
+    out[c++] = key | (unsigned)(nc + __builtin_ctzll(m));
+  const int last = (int)keep[2];
+  for (int o = 0; o <= last; ++o) sp.cf.ent[(size_t)(nc + o) * sp.cf.n + k] = kNoFeat;
+  sp.cf.count[k] = nc + last + 1;
+}
+
 // One thread per sorted slot: counts its items, one atomic per wave reserves
-// the wave's run of the list, then each lane writes its items.  Particles
-// without a usable list get count -1 (k_narrow_full).
+// the wave's run of the list, then each lane writes its items (from the
+// masks of the count pass when the candidates and obstacles fit 64 bits).
+// Particles without a usable list get count -1 (k_narrow_full).
 __global__ __launch_bounds__(256) void k_items(Particles P, Sorted S, const BoxObstacle* __restrict__ obs,
                                                StepParams sp, NbrList L, const DevStatus* st) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool mine = k < sp.n && !st->code;  // after an error: an empty list
   if (mine && L.count[k] > L.cap) sp.cf.count[k] = -1;
-  const int c = mine ? body_items(P, S, obs, sp, L, k, nullptr) : 0;
+  long long keep[3] = {0, 0, -1};
+  const int c = mine ? body_items(P, S, obs, sp, L, k, nullptr, keep) : 0;
   // inclusive scan of the counts over the wave
   const int lane = threadIdx.x & 63;
   int incl = c;
@@ -312,7 +340,11 @@ __global__ __launch_bounds__(256) void k_items(Particles P, Sorted S, const BoxO
   int base = 0;
   if (lane == 63 && total > 0) base = atomicAdd(sp.cf.nitems, total);
   base = __shfl(base, 63);
-  if (mine && L.count[k] <= L.cap) body_items(P, S, obs, sp, L, k, sp.cf.items + base + (incl - c));
+  if (mine && L.count[k] <= L.cap) {
+    unsigned long long* out = sp.cf.items + base + (incl - c);
+    if (L.cap <= 64 && sp.nobs <= 64) write_items(sp, k, L.count[k], keep, out);
+    else body_items(P, S, obs, sp, L, k, out);
+  }
 }
 
 // The work list, one item per lane at a time (a grid of one-wave
